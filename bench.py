@@ -1,0 +1,207 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the spin-tracking hot path on one MI355X.
+
+Workload (the plain-form throughput case): 1,048,576 He-3 atoms, Maxwell-
+Boltzmann velocities at 0.45 K, gas collisions with mean free time 2e-3 s,
+5% diffuse walls, gravity, B0 = (3e-6,0,0) T, E = (0,0,7.5e6) V/m,
+dBx/dz = 1e-8 T/m, DOP853 at rtol = atol = 1e-12, FP64, output every 0.5 s.
+
+One "step" = one output interval: every particle advanced by 0.5 s and its
+snapshot block (the reference's file layout) written to HBM.  --steps K
+times K consecutive intervals (K = 20 is the full 10 s run); --warmup W runs
+W intervals on a separate, freshly seeded ensemble first.  Inputs (the
+seeded ensemble) are resident in HBM before the timed region starts.
+
+Prints ONE JSON line with the metric, a roofline object for the dominant
+kernel (the persistent lane engine) and the CPU oracle timed on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "spin-tracking_amd"))
+
+# Algorithmic FP64 operations per unit of work (each add, sub, mul, div,
+# sqrt or libm call counts 1), for the reference formulation of the path:
+# per accepted DOP853 step (12 right-hand-side evaluations plus stage
+# combinations, error norm and step control), per rejected step, and per
+# event (collision time, move/reflect, sanity check).
+FLOPS_ACC_GRAV = 1410.0
+FLOPS_ACC_NOGRAV = 1200.0
+FLOPS_REJ = 1340.0
+FLOPS_EVENT = 75.0
+FLOPS_RK45_TRIAL = 650.0
+FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 vector, dense
+HBM_PEAK_GBS = 8000.0
+
+
+def build_workload(n, integrator):
+    from spintrack import configs
+    return configs.he3_thermal(n=n, tf=10.0, ioutInt=0.5, seed=1, integrator=integrator)
+
+
+def algorithmic_flops(opt, w):
+    if opt.integratorType == 1:
+        return FLOPS_RK45_TRIAL * (w["accepted"] + w["rejected"]) + FLOPS_EVENT * w["segments"]
+    fa = FLOPS_ACC_GRAV if opt.gravity else FLOPS_ACC_NOGRAV
+    return fa * w["accepted"] + FLOPS_REJ * w["rejected"] + FLOPS_EVENT * w["segments"]
+
+
+def run_gpu(opt, args, targets, persistent, variant):
+    import torch
+    import spintrack as st
+
+    dev = torch.device("cuda:0")
+    # A dedicated (default-priority) stream: the library launches on it and
+    # the HIP events below are recorded on it, so they bracket the kernels.
+    stream = torch.cuda.Stream(dev)
+    assert stream.cuda_stream != 0
+    # warm-up on its own ensemble (first-launch costs, clocks)
+    if args.warmup > 0:
+        wopt = type(opt).from_buffer_copy(bytes(opt))
+        wopt.seed = opt.seed + 7919
+        with st.Particle(wopt, variant=variant) as w:
+            w.set_stream(stream.cuda_stream)
+            w.initParticles()
+            wt = targets[: args.warmup]
+            if persistent:
+                w.run(wt)
+            else:
+                for t in wt:
+                    w.runSimulation(t)
+            torch.cuda.synchronize()
+    p = st.Particle(opt, variant=variant)
+    p.set_stream(stream.cuda_stream)
+    p.initParticles()
+    bb = p.block_bytes()
+    blocks = torch.empty(len(targets) * bb, dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    if persistent:
+        p.run(targets, blocks.data_ptr())
+    else:
+        for k, t in enumerate(targets):
+            p.runSimulation(t)
+            p.snapshot_block_device(blocks.data_ptr() + k * bb)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = ev0.elapsed_time(ev1)
+    w = p.work_counters()
+    fails = int((p.snapshot()["failure"] != 0).sum())
+    p.close()
+    return dict(kernel_ms=kernel_ms, wall_s=wall, work=w, failures=fails)
+
+
+def cpu_baseline(opt, n_total, sample_particles, sample_intervals):
+    """The CPU oracle (the reference CPU path restated in C, OpenMP over
+    particles) on a deterministic subset: every k-th particle of the same
+    ensemble for the first intervals."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as O
+    import spintrack as st
+
+    stride = max(1, n_total // sample_particles)
+    m = n_total // stride
+    o = np.frombuffer(bytes(opt), dtype=O.OPTIONS_DTYPE)[0].copy()
+    o["numParticles"] = m
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    E = O.Ensemble(o, m, first=0, stride=stride)
+    targets = st.output_times(opt)[:sample_intervals]
+    t0 = time.perf_counter()
+    prev = opt.t0
+    for t in targets:
+        E.advance_to(t, prev_t=prev, nthreads=cores)
+        prev = t
+    dt = time.perf_counter() - t0
+    psec = m * (targets[-1] - opt.t0)
+    return dict(value=psec / dt, unit="particle-seconds/s", cores=cores, kind="port",
+                sample=f"every {stride}th particle ({m} particles) for {len(targets)} output "
+                       f"intervals ({targets[-1] - opt.t0:g} s); {dt:.2f} s wall on {cores} threads")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20, help="output intervals timed (20 = full 10 s run)")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--particles", type=int, default=1048576)
+    ap.add_argument("--variant", choices=["fast", "exact"], default="fast")
+    ap.add_argument("--integrator", type=int, default=0, choices=[0, 1])
+    ap.add_argument("--no-plain", action="store_true", help="skip the per-interval (plain) run")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--cpu-sample", type=int, default=8192, help="particles in the CPU sample")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the path shards by particle index, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import spintrack as st
+    opt = build_workload(args.particles, args.integrator)
+    all_targets = st.output_times(opt)
+    steps = max(1, min(args.steps, len(all_targets)))
+    targets = all_targets[:steps]
+    variant = st.VARIANT_FAST if args.variant == "fast" else st.VARIANT_EXACT
+    print(f"[bench] {args.particles} He-3, {steps} intervals of {opt.ioutInt} s, variant={args.variant}",
+          file=sys.stderr, flush=True)
+
+    res = run_gpu(opt, args, targets, True, variant)
+    psec = args.particles * (targets[-1] - opt.t0)
+    value = psec / (res["kernel_ms"] / 1e3)
+    flops = algorithmic_flops(opt, res["work"])
+    achieved_tf = flops / (res["kernel_ms"] / 1e3) / 1e12
+    print(f"[bench] persistent: {res['kernel_ms']:.1f} ms, {value:.4g} particle-s/s, "
+          f"work={res['work']}", file=sys.stderr, flush=True)
+
+    plain = None
+    if not args.no_plain:
+        r2 = run_gpu(opt, args, targets, False, variant)
+        plain = psec / (r2["kernel_ms"] / 1e3)
+        print(f"[bench] plain: {r2['kernel_ms']:.1f} ms, {plain:.4g} particle-s/s", file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(opt, args.particles, args.cpu_sample, min(2, steps))
+        print(f"[bench] cpu: {cpu}", file=sys.stderr, flush=True)
+
+    line = {
+        "metric": "particle-seconds/s, 1 MI355X FP64 rtol=1e-12",
+        "value": value,
+        "unit": "particle-seconds/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": res["kernel_ms"] / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded ensemble, reference RNG)",
+        "config": {"workload": "He-3 thermal 0.45 K, gas tc=2e-3 s, 5% diffuse, gravity, "
+                               "DOP853 rtol=atol=1e-12, output every 0.5 s",
+                   "particles": args.particles, "sim_seconds": targets[-1] - opt.t0,
+                   "variant": args.variant, "integrator": "DOP853" if opt.integratorType == 0 else "RK45",
+                   "path": "persistent (st_run)"},
+        "plain_value": plain,
+        "roofline": {"bound": "mfma", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved_tf / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_engine<fast|exact, DOP853, persistent>",
+                     "algorithmic_flops": flops, "work": res["work"]},
+        "cpu_baseline": cpu,
+        "failures": res["failures"],
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

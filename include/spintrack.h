@@ -1,0 +1,195 @@
+/*
+ * spintrack.h -- C ABI of the MI355X (gfx950) spin-tracking hot path.
+ *
+ * This is the drop-in boundary for the reference's simulation loop.  The
+ * reference has no plugin API; its loop is the `particle` class
+ * (include/particle.h:87-314) driven by mainAnalysis (src/simulation.cpp:8-97):
+ *
+ *   particle p(opt);            -> st_create(&opt, NULL, &h)
+ *   p.initParticles();          -> st_init(h)
+ *   p.runSimulation(nextTOut);  -> st_advance_to(h, nextTOut)
+ *   p.outputData(log);          -> st_snapshot(h, t, pos, v, S, failure,
+ *                                              n_coll, n_bounce, n_steps)
+ *   ~particle()                 -> st_destroy(h)
+ *
+ * Conventions: every entry point returns 0 on success and a negative ST_E*
+ * code on failure (never exit()s, never throws across the ABI); the message
+ * of the last failure is available from st_last_error().  Per-particle
+ * failures keep the reference's convention (failureState 1-4 from the
+ * sanity check, negative integrator codes, NaN-poisoned t/pos/v/S;
+ * src/particle.cpp:412-444, 596-612).  All pointers passed to the st_*
+ * calls are host pointers unless the name ends in _device.  One handle must
+ * not be used from two host threads at once (the reference's particle
+ * object is not re-entrant either).
+ */
+#ifndef SPINTRACK_H
+#define SPINTRACK_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ST_API __attribute__((visibility("default")))
+
+/* error codes */
+#define ST_OK            0
+#define ST_E_ARG        -1   /* invalid argument (null pointer, bad size, ...) */
+#define ST_E_HIP        -2   /* a HIP runtime call failed */
+#define ST_E_UNSUPPORTED -3  /* an option this build does not implement */
+#define ST_E_STATE      -4   /* call out of order (e.g. advance before init) */
+
+/* include/double3.h:17-21 */
+typedef struct st_coords { double x, y, z; } st_coords;
+
+/* include/options.h:10-61.  Byte-identical layout (440 bytes on x86-64 and
+ * on gfx950); it is also the header of the reference binary output format
+ * (src/logger.cpp:27-29). */
+typedef struct st_options {
+    st_coords B0, E, L, yi, Gx, Gy, Gz;
+    double m, t0, tf, rtol, atol, beta, uround, safe, fac1, fac2, hmax, hmin,
+           h, T, sqrtKT_m, tc, gamma, V, a, w, swapStepSize, maxPosStep,
+           ioutInt, diffuse;
+    uint32_t nmax, seed;
+    int32_t integratorType, numParticles, numPerGPUBlock, iout;
+    char dist;
+    uint8_t gas_coll, gravity, fixedStepSize, keepStepSize;
+    st_coords noiseAmplitudes, noiseFrequencies;
+} st_options;
+
+/* Knobs the reference hard-codes in its user-editable hooks, plus the
+ * choice of arithmetic variant.  Pass NULL for the reference behaviour. */
+#define ST_VARIANT_EXACT 0  /* the reference's operation order, no FMA contraction */
+#define ST_VARIANT_FAST  1  /* FMA-contracted, per-segment field polynomial; agrees
+                               with EXACT to the integrator tolerance */
+typedef struct st_ext {
+    int32_t pulse_axis;     /* axis of pulse(t) = a*cos(w*t): 0,1,2; reference = 2 (z),
+                               src/integrator.cpp:40-42 */
+    int32_t variant;        /* ST_VARIANT_* */
+    int32_t block_size;     /* threads per workgroup for the plain kernel; 0 = default */
+    int32_t reserved;
+} st_ext;
+
+/* include/particle.h:52-59 */
+typedef struct st_rng {
+    uint64_t x, y, z, w;
+    double spare;
+    uint8_t hasSpare;
+    uint8_t pad_[7];
+} st_rng;
+
+/* Complete per-particle state (everything the reference keeps across
+ * runSimulation calls, include/particle.h:147-170), for checkpoints and tests. */
+typedef struct st_particle {
+    st_coords S, v, v_old, pos, pos_old;
+    double t, t_old, tf, next_gas, h;
+    st_rng rng;
+    uint64_t n_bounce, n_coll, n_steps;
+    int32_t failure;
+    uint32_t part_id;
+    uint8_t stop, coll_type, wall_hit, pad_[5];
+} st_particle;
+
+typedef struct st_handle st_handle;
+
+/* == particle::particle(opt) (include/particle.h:90-166): validates the
+ * options and allocates the device state for opt->numParticles particles in
+ * HBM.  opt->tc and opt->sqrtKT_m are used as given (the reference's
+ * mainAnalysis derives them, src/simulation.cpp:9-11).  ext may be NULL. */
+ST_API int st_create(const st_options* opt, const st_ext* ext, st_handle** out);
+
+/* == particle::~particle (include/particle.h:168-240) */
+ST_API void st_destroy(st_handle* h);
+
+/* Message of the most recent failure on this handle (or of the last failed
+ * st_create when h is NULL). */
+ST_API const char* st_last_error(const st_handle* h);
+
+/* Run all later work of this handle on the given hipStream_t (NULL = the
+ * handle's own stream).  The stream must have the default priority. */
+ST_API int st_set_stream(st_handle* h, void* hip_stream);
+
+/* == particle::initParticles (include/particle.h:241-252, initParticlesCPU
+ * src/particle.cpp:638-710): seeds particle i from splitmix64(seed + i). */
+ST_API int st_init(st_handle* h);
+
+/* Seeds local slot j as global particle first + j*stride, so a deterministic
+ * subset of a larger ensemble can be replayed.  st_init == (0, 1). */
+ST_API int st_init_subset(st_handle* h, uint32_t first, uint32_t stride);
+
+/* == particle::runSimulation(nextTOut) (src/particle.cpp:865-876 ->
+ * runSimulationGPU :525-635): advances every particle to next_t.
+ * Asynchronous on the handle's stream; st_synchronize() or any copy-out
+ * call waits for it. */
+ST_API int st_advance_to(st_handle* h, double next_t);
+
+/* == particle::outputData -> BinaryLogger::writeSnapshot
+ * (src/particle.cpp:852-862, src/logger.cpp:14-25): copies the eight
+ * snapshot arrays to host memory in the reference's layout.  Any pointer may
+ * be NULL to skip that array. */
+ST_API int st_snapshot(st_handle* h, double* t, st_coords* pos, st_coords* v,
+                       st_coords* S, int32_t* failure, uint64_t* n_coll,
+                       uint64_t* n_bounce, uint64_t* n_steps);
+
+/* Bytes of one snapshot block in the reference file layout: numParticles *
+ * (8 + 24 + 24 + 24 + 4 + 8 + 8 + 8) = 108 * numParticles. */
+ST_API size_t st_snapshot_block_bytes(const st_handle* h);
+
+/* Writes the current state as one snapshot block (the byte image
+ * BinaryLogger::writeSnapshot writes) into device memory at dst. */
+ST_API int st_snapshot_block_device(st_handle* h, void* dst);
+
+/* Persistent multi-interval run: advances every particle through the
+ * output times targets[0..n_targets-1] (each > the previous) in one
+ * launch, with a work queue that refills lanes as particles finish.  When
+ * blocks_dev is not NULL, snapshot block k (the state at targets[k]) is
+ * written to blocks_dev + k*st_snapshot_block_bytes(h).  The per-particle
+ * results are identical to calling st_advance_to(targets[k]) in order. */
+ST_API int st_run(st_handle* h, int32_t n_targets, const double* targets,
+                  void* blocks_dev);
+
+/* Per-particle end times (batched extension; the reference overwrites its
+ * per-particle tf array, src/particle.cpp:748-749).  Particle i takes part
+ * in an interval (t_prev, t_next] only while t_prev < stop_at[i], and then
+ * advances to min(t_next, stop_at[i]).  NULL clears it. */
+ST_API int st_set_stop_times(st_handle* h, const double* stop_at);
+
+/* Full state copy-out / copy-in (n = numParticles records). */
+ST_API int st_get_state(st_handle* h, st_particle* out);
+ST_API int st_set_state(st_handle* h, const st_particle* in);
+
+/* ODE work counters summed since st_init: accepted trial steps, rejected
+ * trial steps, integrated segments, and the number of event-loop trips.
+ * (The reference keeps no ODE step counter, src/integrator.cpp:101.) */
+ST_API int st_work_counters(st_handle* h, uint64_t out[4]);
+
+/* Per-particle ODE work since st_init (n = numParticles entries each; any
+ * pointer may be NULL): accepted and rejected trial steps, and integrated
+ * segments. */
+ST_API int st_particle_counters(st_handle* h, uint64_t* accepted, uint64_t* rejected,
+                                uint64_t* segments);
+
+/* Segment trace of one particle (tracing/debug facility; the reference has
+ * none).  particle < 0 disables.  Later advance/run calls append two
+ * records per integrated segment of that particle, ST_TRACE_DOUBLES doubles
+ * each: tag (1 = segment start, 2 = end), t_old, t, pos_old[3], v_old[3],
+ * S[3], step size, accepted, rejected, coll_type, wall_hit, integrator
+ * status, n_bounce, n_coll.  st_get_trace copies up to capacity records and
+ * reports how many were recorded (at most the capacity given to
+ * st_set_trace). */
+#define ST_TRACE_DOUBLES 20
+ST_API int st_set_trace(st_handle* h, int64_t particle, uint32_t capacity);
+ST_API int st_get_trace(st_handle* h, double* out, uint32_t capacity, uint32_t* n_records);
+
+/* Waits for all work queued on the handle's stream. */
+ST_API int st_synchronize(st_handle* h);
+
+/* Library identification: "spintrack gfx950 <variant list>". */
+ST_API const char* st_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,488 @@
+// st_device.h -- per-lane device math of the spin-tracking hot path (gfx950).
+//
+// One particle per lane.  Everything here is FP64 on the vector ALUs.  The
+// translation unit is compiled with -ffp-contract=off, so the "exact"
+// functions round every +,-,*,/ exactly as the reference's x86-64 build
+// does, in the reference's operand order (its coords operators are
+// left-associative, src/double3.cpp:21-145).  The "fast" functions opt back
+// into FMA contraction with a scoped pragma and restructure the right-hand
+// side around a per-segment field polynomial; they agree with the exact
+// path to the integrator tolerance, not bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+// Device math is also compiled for the host so a CPU build of the same
+// functions can be checked against the oracle without a GPU.
+#define ST_HD __host__ __device__
+
+namespace st {
+
+// ---------------------------------------------------------------------------
+// constants (include/options.h:7-8)
+// ---------------------------------------------------------------------------
+constexpr double G_CONST = -9.81;
+constexpr double HALF_G = 0.5 * G_CONST;                 // 0.5*a.y, exact
+constexpr double C2 = 299792458.0 * 299792458.0;
+constexpr double IC2 = 1.0 / C2;                          // the reference's 1.0/c2
+constexpr double PI_D = 3.14159265358979323846;           // M_PI
+constexpr double FLT_MIN_D = 1.17549435082228750797e-38;  // FLT_MIN
+constexpr double DBL_MAX_D = 1.79769313486231570815e+308; // DBL_MAX
+
+// arithmetic variants of the lane engine
+enum { VAR_EXACT = 0, VAR_FAST = 1 };
+// integrator types (options.integratorType; src/particle.cpp:568-592)
+enum { INT_NONE = -1, INT_DOP853 = 0, INT_RK45 = 1 };
+
+struct d3 {
+    double x, y, z;
+};
+
+ST_HD __forceinline__ d3 mk3(double x, double y, double z) { return d3{x, y, z}; }
+
+// src/double3.cpp:85-91
+ST_HD __forceinline__ d3 cross3(const d3& a, const d3& b) {
+    return d3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+// src/double3.cpp:93-95
+ST_HD __forceinline__ double dot3(const d3& a, const d3& b) {
+    return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+// src/double3.cpp:101-103
+ST_HD __forceinline__ double len3(const d3& a) {
+    return sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
+}
+
+// src/particle.cpp:24-31: sgn(0) = +1
+ST_HD __forceinline__ double psgn(double v) { return (v < 0) ? -1.0 : 1.0; }
+// std::min / std::max as called by the reference
+ST_HD __forceinline__ double std_min(double a, double b) { return (b < a) ? b : a; }
+ST_HD __forceinline__ double std_max(double a, double b) { return (a < b) ? b : a; }
+// src/integrator.cpp:20-33
+ST_HD __forceinline__ double sign_d(double a, double b) { return (b < 0.0) ? -fabs(a) : fabs(a); }
+ST_HD __forceinline__ double min_d(double a, double b) { return (a < b) ? a : b; }
+ST_HD __forceinline__ double max_d(double a, double b) { return (a > b) ? a : b; }
+
+// ---------------------------------------------------------------------------
+// Uniform (SGPR-resident) run parameters, derived once on the host from the
+// reference options block.
+// ---------------------------------------------------------------------------
+struct SimParams {
+    double Lx, Ly, Lz;
+    double B0x, B0y, B0z;
+    double Ex, Ey, Ez;
+    double G[9];          // rows Gx, Gy, Gz (grad, src/integrator.cpp:44-46)
+    double gamma;
+    double pa, pw;        // pulse amplitude and angular frequency
+    double rtol, atol, beta, safe, fac1, fac2, hmax, hmin, h0;
+    double expo1, facc1, facc2;   // integrateDOP set-up (src/integrator.cpp:105-108)
+    double maxPosStep, tc, sqrtKT_m, diffuse;
+    uint32_t nmax;
+    int32_t pulse_axis;
+    int32_t pulse_on;     // 0: the pulse term is +0 and may be skipped bit-exactly
+    int32_t pow_beta_on;  // 0: beta == 0, pow(facold, beta) == 1 exactly
+    int32_t dist;         // 'C' or 'M'
+    int32_t gravity, gas_coll, fixed, keep, diffuse_on;
+};
+
+// ---------------------------------------------------------------------------
+// RNG (src/particle.cpp:33-119)
+// ---------------------------------------------------------------------------
+struct Rng {
+    uint64_t x, y, z, w;
+    double spare;
+    bool has;
+};
+
+// The reference's xoshiro variant (src/particle.cpp:59-70): output x+z, and
+// w becomes rotl(z, 45) instead of rotl(w, 45).
+ST_HD __forceinline__ uint64_t xo_next(Rng& s) {
+    const uint64_t r = s.x + s.z;
+    const uint64_t t = s.y << 17;
+    s.z ^= s.x;
+    s.w ^= s.y;
+    s.y ^= s.z;
+    s.x ^= s.w;
+    s.z ^= t;
+    s.w = (s.z << 45) | (s.z >> 19);
+    return r;
+}
+// src/particle.cpp:72-83 (53-bit integer -> double is exact)
+ST_HD __forceinline__ double uni(Rng& s) { return (double)(xo_next(s) >> 11) * 0x1.0p-53; }
+// src/particle.cpp:85-87
+ST_HD __forceinline__ double uni_lh(Rng& s, double lo, double hi) { return uni(s) * (hi - lo) + lo; }
+
+// src/particle.cpp:89-107: Marsaglia polar method, spare cached in the state
+ST_HD inline double normal(Rng& s, double mean, double sd) {
+    if (s.has) {
+        s.has = false;
+        return s.spare * sd + mean;
+    }
+    double ts, tu, tv;
+    do {
+        tu = uni_lh(s, -1.0, 1.0);
+        tv = uni_lh(s, -1.0, 1.0);
+        ts = tu * tu + tv * tv;
+    } while (ts >= 1.0 || ts == 0.0);
+    ts = sqrt(-2.0 * log(ts) / ts);
+    s.spare = tv * ts;
+    s.has = true;
+    return mean + sd * tu * ts;
+}
+ST_HD __forceinline__ double maxboltz(Rng& s, double sqrtkT_m) { return normal(s, 0.0, 1.0) * sqrtkT_m; }
+// Three consecutive normal(0,1) draws through ONE copy of the polar-method
+// code (a rolled loop keeps the event handler small).  normal(s, 0, 1)
+// returns 0 + (1*tu)*ts or spare*1 + 0, i.e. exactly tu*ts or spare.
+ST_HD __forceinline__ void normal3(Rng& s, double& n0, double& n1, double& n2) {
+    double out0 = 0.0, out1 = 0.0, out2 = 0.0;
+#pragma unroll 1
+    for (int c = 0; c < 3; ++c) {
+        const double g = normal(s, 0.0, 1.0);
+        if (c == 0) out0 = g;
+        else if (c == 1) out1 = g;
+        else out2 = g;
+    }
+    n0 = out0;
+    n1 = out1;
+    n2 = out2;
+}
+ST_HD __forceinline__ double unif02pi(Rng& s) { return uni(s) * 2.0 * PI_D; }
+ST_HD __forceinline__ double expo_draw(Rng& s, double tc) { return -tc * log(1.0 - uni(s)); }
+
+// src/particle.cpp:646-663 / 455-473: splitmix64(seed + index), the index
+// sum taken in 32-bit unsigned arithmetic, split into 32-bit halves, then
+// three discarded draws.
+ST_HD inline void seed_rng(Rng& s, uint32_t seed, uint32_t ipart) {
+    uint64_t st = (uint64_t)(uint32_t)(seed + ipart);
+    uint64_t r = (st += 0x9E3779B97f4A7C15ull);
+    r = (r ^ (r >> 30)) * 0xBF58476D1CE4E5B9ull;
+    r = (r ^ (r >> 27)) * 0x94D049BB133111EBull;
+    r = r ^ (r >> 31);
+    s.x = (uint32_t)r;
+    s.y = (uint32_t)(r >> 32);
+    r = (st += 0x9E3779B97f4A7C15ull);
+    r = (r ^ (r >> 30)) * 0xBF58476D1CE4E5B9ull;
+    r = (r ^ (r >> 27)) * 0x94D049BB133111EBull;
+    r = r ^ (r >> 31);
+    s.z = (uint32_t)r;
+    s.w = (uint32_t)(r >> 32);
+    s.spare = 0.0;
+    s.has = false;
+    (void)uni(s);
+    (void)uni(s);
+    (void)uni(s);
+}
+
+// ---------------------------------------------------------------------------
+// Kinematics (src/particle.cpp:121-444)
+// ---------------------------------------------------------------------------
+struct Kin {
+    double t, t_old, next_gas, dt;
+    d3 pos, pos_old, v, v_old;
+    uint64_t n_bounce, n_coll;
+    int32_t failure;
+    bool stop, finished;
+    char coll_type, wall_hit;
+};
+
+// src/particle.cpp:121-257 -- closed-form wall hit times (quadratic in y
+// under gravity), the else-if fix-up chain, tie rules, gas vs wall, and the
+// maxPosStep / interval-end cap.
+ST_HD inline void next_collision(const SimParams& P, double tf, Kin& K, Rng& rng) {
+    const d3 v = K.v, pos = K.pos;
+    const double t = K.t;
+    double dx, dy, dz, dtx, dty, dtz;
+    if (P.gravity) {
+        dx = psgn(v.x) * P.Lx / 2.0 - pos.x;
+        dz = psgn(v.z) * P.Lz / 2.0 - pos.z;
+        dtx = dx / v.x;
+        dtz = dz / v.z;
+        const double y2 = v.y * v.y;
+        if (psgn(v.y) <= 0.0) {
+            dy = pos.y + P.Ly * 0.5;
+            const double sqr = sqrt(-2.0 * G_CONST * dy + y2);
+            const double temp1 = -(sqr + v.y) / G_CONST;
+            const double temp2 = (sqr - v.y) / G_CONST;
+            dty = std_min(fabs(temp1), fabs(temp2));
+        } else {
+            const double maxHeight = -0.5 * y2 / G_CONST + pos.y;
+            if (maxHeight < 0.5 * P.Ly) {
+                const double t1 = -v.y / G_CONST;
+                const double top = pos.y + v.y * t1 + 0.5 * G_CONST * t1 * t1;
+                const double t2 = sqrt((-0.5 * P.Ly - top) * 2.0 / G_CONST);
+                dty = t1 + t2;
+            } else {
+                const double sqr = sqrt(y2 - 2.0 * G_CONST * (pos.y - (0.5 * P.Ly)));
+                const double temp1 = (-v.y + sqr) / G_CONST;
+                const double temp2 = (-v.y - sqr) / G_CONST;
+                dty = std_min(fabs(temp1), fabs(temp2));
+            }
+        }
+        if (dtx < 1e-16 || __builtin_isnan(dtx))
+            dtx = 1e6;
+        else if (dty < 1e-16 || __builtin_isnan(dty))
+            dty = 1e6;
+        else if (dtz < 1e-16 || __builtin_isnan(dtz))
+            dtz = 1e6;
+    } else {
+        dx = psgn(v.x) * P.Lx / 2.0 - pos.x;
+        dy = psgn(v.y) * P.Ly / 2.0 - pos.y;
+        dz = psgn(v.z) * P.Lz / 2.0 - pos.z;
+        dtx = dx / v.x;
+        dty = dy / v.y;
+        dtz = dz / v.z;
+        if (dtx < 1e-16)
+            dtx = 1e6;
+        else if (dty < 1e-16)
+            dty = 1e6;
+        else if (dtz < 1e-16)
+            dtz = 1e6;
+    }
+    // Uninitialised in the reference when no branch matches (NaN times only).
+    double tb = __builtin_nan("");
+    char wh = K.wall_hit;
+    if (dtx < dty && dtx < dtz) {
+        tb = dtx; wh = 'x';
+    } else if (dty < dtx && dty < dtz) {
+        tb = dty; wh = 'y';
+    } else if (dtz < dtx && dtz < dty) {
+        tb = dtz; wh = 'z';
+    } else if (dtx == dty && dtx == dtz) {
+        tb = dtx; wh = 'a';
+    } else if (dtx == dty) {
+        tb = dtx; wh = 'b';
+    } else if (dtx == dtz) {
+        tb = dtx; wh = 'c';
+    } else if (dty == dtz) {
+        tb = dty; wh = 'd';
+    }
+    K.wall_hit = wh;
+    const double tgas = K.next_gas - t;
+    bool gas_next;
+    double tcol;
+    if (tgas < tb) {
+        gas_next = true;
+        tcol = tgas;
+    } else if (tgas == tb) {
+        K.next_gas += expo_draw(rng, P.tc);
+        tcol = tgas;
+        gas_next = false;
+    } else {
+        gas_next = false;
+        tcol = tb;
+    }
+    bool reach_end;
+    double tnone;
+    if (tf - t <= P.maxPosStep) {
+        reach_end = true;
+        tnone = tf - t;
+    } else {
+        reach_end = false;
+        tnone = P.maxPosStep;
+    }
+    if (tcol <= tnone) {
+        K.dt = tcol;
+        if (gas_next) {
+            K.coll_type = 'G';
+            K.n_coll += 1;
+            K.next_gas += expo_draw(rng, P.tc);
+        } else {
+            K.n_bounce += 1;
+            K.coll_type = 'W';
+        }
+        if (tcol == tnone && reach_end) K.finished = true;
+    } else {
+        K.coll_type = 'N';
+        K.dt = tnone;
+        if (reach_end) K.finished = true;
+    }
+}
+
+ST_HD __forceinline__ void flip_v(d3& v, char wh) {
+    // src/particle.cpp:329-371
+    if (wh == 'x' || wh == 'a' || wh == 'b' || wh == 'c') v.x *= -1.0;
+    if (wh == 'y' || wh == 'a' || wh == 'b' || wh == 'd') v.y *= -1.0;
+    if (wh == 'z' || wh == 'a' || wh == 'c' || wh == 'd') v.z *= -1.0;
+}
+
+// src/particle.cpp:263-388
+ST_HD inline void move_and_collide(const SimParams& P, Kin& K, Rng& rng) {
+    K.v_old = K.v;
+    K.t_old = K.t;
+    K.t += K.dt;
+    K.pos_old = K.pos;
+    const double d = K.dt;
+    const double ay = P.gravity ? G_CONST : 0.0;
+    // pos = pos_old + v*dt + 0.5*a*dt*dt with a = (0, ay, 0); the x and z
+    // gravity terms are (0*dt)*dt == +0 for finite dt.
+    K.pos.x = (K.pos_old.x + K.v.x * d) + 0.0;
+    K.pos.y = (K.pos_old.y + K.v.y * d) + ((0.5 * ay) * d) * d;
+    K.pos.z = (K.pos_old.z + K.v.z * d) + 0.0;
+    K.v.x = K.v.x + 0.0;
+    K.v.y = K.v.y + ay * d;
+    K.v.z = K.v.z + 0.0;
+    const double Vel = len3(K.v);
+    const char ct = K.coll_type, wh = K.wall_hit;
+    if (ct == 'W') {
+        if (wh == 'x' || wh == 'a' || wh == 'b' || wh == 'c') K.pos.x = psgn(K.pos.x) * P.Lx * 0.5;
+        if (wh == 'y' || wh == 'a' || wh == 'b' || wh == 'd') K.pos.y = psgn(K.pos.y) * P.Ly * 0.5;
+        if (wh == 'z' || wh == 'a' || wh == 'c' || wh == 'd') K.pos.z = psgn(K.pos.z) * P.Lz * 0.5;
+        bool diffuse = false;
+        if (P.diffuse_on) diffuse = uni(rng) < P.diffuse;
+        if (diffuse) {
+            // Lambertian re-emission at the post-flight speed; the reference
+            // draws phi and theta before it looks at the wall id.
+            const double phi = acos(sqrt(uni(rng)));
+            const double theta = unif02pi(rng);
+            const double sp = sin(phi), cp = cos(phi), cth = cos(theta), sth = sin(theta);
+            if (wh == 'x') {
+                K.v.x = -1.0 * psgn(K.v.x) * Vel * cp;
+                K.v.y = -Vel * sp * cth;
+                K.v.z = Vel * sp * sth;
+            } else if (wh == 'y') {
+                K.v.x = Vel * sp * cth;
+                K.v.y = -1.0 * psgn(K.v.y) * Vel * cp;
+                K.v.z = Vel * sp * sth;
+            } else if (wh == 'z') {
+                K.v.x = Vel * sp * cth;
+                K.v.y = Vel * sp * sth;
+                K.v.z = -1.0 * psgn(K.v.z) * Vel * cp;
+            } else {
+                K.stop = true;
+            }
+        } else {
+            flip_v(K.v, wh);
+        }
+    } else if (ct == 'G') {
+        // 'M': three Maxwell-Boltzmann components; 'C': isotropic direction at
+        // the current speed.  Both draw three normals in the same order.
+        if (P.dist == 'M' || P.dist == 'C') {
+            double n0, n1, n2;
+            normal3(rng, n0, n1, n2);
+            if (P.dist == 'M') {
+                K.v.x = n0 * P.sqrtKT_m;
+                K.v.y = n1 * P.sqrtKT_m;
+                K.v.z = n2 * P.sqrtKT_m;
+            } else {
+                const double vn = len3(d3{n0, n1, n2});
+                K.v.x = (Vel * n0) / vn;
+                K.v.y = (Vel * n1) / vn;
+                K.v.z = (Vel * n2) / vn;
+            }
+        }
+    }
+}
+
+// src/particle.cpp:412-444
+ST_HD inline void sanity(const SimParams& P, Kin& K) {
+    const double tol = 1.0e-15;
+    if (fabs(K.pos.x) > P.Lx / 2.0 + tol || fabs(K.pos.y) > P.Ly / 2.0 + tol ||
+        fabs(K.pos.z) > P.Lz / 2.0 + tol) {
+        K.stop = true; K.failure = 1; return;
+    }
+    if (__builtin_isinf(K.t_old) || __builtin_isinf(K.t)) { K.stop = true; K.failure = 2; return; }
+    if (__builtin_isnan(K.pos_old.x) || __builtin_isnan(K.pos_old.y) || __builtin_isnan(K.pos_old.z) ||
+        __builtin_isnan(K.pos.x) || __builtin_isnan(K.pos.y) || __builtin_isnan(K.pos.z)) {
+        K.stop = true; K.failure = 3; return;
+    }
+    if (__builtin_isnan(K.v.x) || __builtin_isnan(K.v.y) || __builtin_isnan(K.v.z) ||
+        __builtin_isnan(K.v_old.x) || __builtin_isnan(K.v_old.y) || __builtin_isnan(K.v_old.z)) {
+        K.failure = 4; K.stop = true; return;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Bloch right-hand side along a ballistic segment (src/integrator.cpp:40-88)
+// ---------------------------------------------------------------------------
+struct Seg {
+    double t0;     // segment start time (t_old)
+    d3 p0, v0;     // position and velocity at t0 (pos_old, v_old)
+};
+
+// Exact: interpolate -> grad -> pulse + B0 + (1/c^2) v x E + G.r -> gamma*B ->
+// y x (gamma B), in the reference's order.
+ST_HD __forceinline__ d3 rhs_exact(const SimParams& P, double t, const d3& y, const Seg& S) {
+    const double d = t - S.t0;
+    d3 p, v;
+    if (P.gravity) {
+        p.x = (S.p0.x + S.v0.x * d) + 0.0;
+        p.y = (S.p0.y + S.v0.y * d) + (HALF_G * d) * d;
+        p.z = (S.p0.z + S.v0.z * d) + 0.0;
+        v.x = S.v0.x + 0.0;
+        v.y = S.v0.y + G_CONST * d;
+        v.z = S.v0.z + 0.0;
+    } else {
+        p.x = S.p0.x + S.v0.x * d;
+        p.y = S.p0.y + S.v0.y * d;
+        p.z = S.p0.z + S.v0.z * d;
+        v = S.v0;
+    }
+    const double gx = p.x * P.G[0] + p.y * P.G[1] + p.z * P.G[2];
+    const double gy = p.x * P.G[3] + p.y * P.G[4] + p.z * P.G[5];
+    const double gz = p.x * P.G[6] + p.y * P.G[7] + p.z * P.G[8];
+    double plx = 0.0, ply = 0.0, plz = 0.0;
+    if (P.pulse_on) {
+        const double pv = P.pa * cos(P.pw * t);
+        if (P.pulse_axis == 0) plx = pv;
+        else if (P.pulse_axis == 1) ply = pv;
+        else plz = pv;
+    }
+    const double cx = v.y * P.Ez - v.z * P.Ey;
+    const double cy = v.z * P.Ex - v.x * P.Ez;
+    const double cz = v.x * P.Ey - v.y * P.Ex;
+    const double Bx = ((plx + P.B0x) + cx * IC2) + gx;
+    const double By = ((ply + P.B0y) + cy * IC2) + gy;
+    const double Bz = ((plz + P.B0z) + cz * IC2) + gz;
+    const d3 w{Bx * P.gamma, By * P.gamma, Bz * P.gamma};
+    return cross3(y, w);
+}
+
+// Fast: along a segment gamma*B is a quadratic polynomial in tau = t - t0
+// (position is quadratic and velocity linear in tau, every field term is
+// linear in them); the pulse, when on, is added separately.
+struct Poly {
+    d3 w0, w1, w2;
+};
+
+ST_HD inline Poly make_poly(const SimParams& P, const Seg& S) {
+#pragma clang fp contract(fast)
+    const double ay = P.gravity ? G_CONST : 0.0;
+    // B0 + G.p0 + (1/c^2) v0 x E
+    const double g0x = S.p0.x * P.G[0] + S.p0.y * P.G[1] + S.p0.z * P.G[2];
+    const double g0y = S.p0.x * P.G[3] + S.p0.y * P.G[4] + S.p0.z * P.G[5];
+    const double g0z = S.p0.x * P.G[6] + S.p0.y * P.G[7] + S.p0.z * P.G[8];
+    const double c0x = S.v0.y * P.Ez - S.v0.z * P.Ey;
+    const double c0y = S.v0.z * P.Ex - S.v0.x * P.Ez;
+    const double c0z = S.v0.x * P.Ey - S.v0.y * P.Ex;
+    // G.v0 + (1/c^2) a x E, a = (0, ay, 0)
+    const double g1x = S.v0.x * P.G[0] + S.v0.y * P.G[1] + S.v0.z * P.G[2];
+    const double g1y = S.v0.x * P.G[3] + S.v0.y * P.G[4] + S.v0.z * P.G[5];
+    const double g1z = S.v0.x * P.G[6] + S.v0.y * P.G[7] + S.v0.z * P.G[8];
+    const double c1x = ay * P.Ez, c1z = -ay * P.Ex;
+    Poly q;
+    const double gm = P.gamma;
+    q.w0 = d3{gm * (P.B0x + g0x + c0x * IC2), gm * (P.B0y + g0y + c0y * IC2),
+              gm * (P.B0z + g0z + c0z * IC2)};
+    q.w1 = d3{gm * (g1x + c1x * IC2), gm * g1y, gm * (g1z + c1z * IC2)};
+    // 0.5 * G.a
+    q.w2 = d3{gm * (0.5 * ay * P.G[1]), gm * (0.5 * ay * P.G[4]), gm * (0.5 * ay * P.G[7])};
+    return q;
+}
+
+ST_HD __forceinline__ d3 rhs_fast(const SimParams& P, double t, const d3& y, double t0,
+                                       const Poly& q) {
+#pragma clang fp contract(fast)
+    const double tau = t - t0;
+    d3 w{q.w0.x + tau * (q.w1.x + tau * q.w2.x), q.w0.y + tau * (q.w1.y + tau * q.w2.y),
+         q.w0.z + tau * (q.w1.z + tau * q.w2.z)};
+    if (P.pulse_on) {
+        const double pv = P.gamma * (P.pa * cos(P.pw * t));
+        if (P.pulse_axis == 0) w.x += pv;
+        else if (P.pulse_axis == 1) w.y += pv;
+        else w.z += pv;
+    }
+    return cross3(y, w);
+}
+
+}  // namespace st

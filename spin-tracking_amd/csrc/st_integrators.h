@@ -1,0 +1,507 @@
+// st_integrators.h -- one trial step of DOP853 / RK45 per call, so that a
+// lane can interleave trial steps with event handling in one flat loop.
+//
+// integrateDOP (src/integrator.cpp:90-249) and integrateRK45 (:251-337)
+// loop over trial steps until the segment end is reached; here the loop
+// state lives in DopState / RkState and each call performs exactly one
+// iteration of the reference's while(1) body.  The sequence of operations
+// per iteration is the reference's.
+#pragma once
+
+#include "st_device.h"
+
+namespace st {
+
+// ---------------------------------------------------------------------------
+// DOP853 tableau (include/coeff.h:6-176; Hairer & Wanner's DOP853)
+// ---------------------------------------------------------------------------
+namespace dop {
+constexpr double c2 = 0.526001519587677318785587544488E-01;
+constexpr double c3 = 0.789002279381515978178381316732E-01;
+constexpr double c4 = 0.118350341907227396726757197510E+00;
+constexpr double c5 = 0.281649658092772603273242802490E+00;
+constexpr double c6 = 0.333333333333333333333333333333E+00;
+constexpr double c7 = 0.25E+00;
+constexpr double c8 = 0.307692307692307692307692307692E+00;
+constexpr double c9 = 0.651282051282051282051282051282E+00;
+constexpr double c10 = 0.6E+00;
+constexpr double c11 = 0.857142857142857142857142857142E+00;
+constexpr double b1 = 5.42937341165687622380535766363E-2;
+constexpr double b6 = 4.45031289275240888144113950566E0;
+constexpr double b7 = 1.89151789931450038304281599044E0;
+constexpr double b8 = -5.8012039600105847814672114227E0;
+constexpr double b9 = 3.1116436695781989440891606237E-1;
+constexpr double b10 = -1.52160949662516078556178806805E-1;
+constexpr double b11 = 2.01365400804030348374776537501E-1;
+constexpr double b12 = 4.47106157277725905176885569043E-2;
+constexpr double bhh1 = 0.244094488188976377952755905512E+00;
+constexpr double bhh2 = 0.733846688281611857341361741547E+00;
+constexpr double bhh3 = 0.220588235294117647058823529412E-01;
+constexpr double er1 = 0.1312004499419488073250102996E-01;
+constexpr double er6 = -0.1225156446376204440720569753E+01;
+constexpr double er7 = -0.4957589496572501915214079952E+00;
+constexpr double er8 = 0.1664377182454986536961530415E+01;
+constexpr double er9 = -0.3503288487499736816886487290E+00;
+constexpr double er10 = 0.3341791187130174790297318841E+00;
+constexpr double er11 = 0.8192320648511571246570742613E-01;
+constexpr double er12 = -0.2235530786388629525884427845E-01;
+constexpr double a21 = 5.26001519587677318785587544488E-2;
+constexpr double a31 = 1.97250569845378994544595329183E-2;
+constexpr double a32 = 5.91751709536136983633785987549E-2;
+constexpr double a41 = 2.95875854768068491816892993775E-2;
+constexpr double a43 = 8.87627564304205475450678981324E-2;
+constexpr double a51 = 2.41365134159266685502369798665E-1;
+constexpr double a53 = -8.84549479328286085344864962717E-1;
+constexpr double a54 = 9.24834003261792003115737966543E-1;
+constexpr double a61 = 3.7037037037037037037037037037E-2;
+constexpr double a64 = 1.70828608729473871279604482173E-1;
+constexpr double a65 = 1.25467687566822425016691814123E-1;
+constexpr double a71 = 3.7109375E-2;
+constexpr double a74 = 1.70252211019544039314978060272E-1;
+constexpr double a75 = 6.02165389804559606850219397283E-2;
+constexpr double a76 = -1.7578125E-2;
+constexpr double a81 = 3.70920001185047927108779319836E-2;
+constexpr double a84 = 1.70383925712239993810214054705E-1;
+constexpr double a85 = 1.07262030446373284651809199168E-1;
+constexpr double a86 = -1.53194377486244017527936158236E-2;
+constexpr double a87 = 8.27378916381402288758473766002E-3;
+constexpr double a91 = 6.24110958716075717114429577812E-1;
+constexpr double a94 = -3.36089262944694129406857109825E0;
+constexpr double a95 = -8.68219346841726006818189891453E-1;
+constexpr double a96 = 2.75920996994467083049415600797E1;
+constexpr double a97 = 2.01540675504778934086186788979E1;
+constexpr double a98 = -4.34898841810699588477366255144E1;
+constexpr double a101 = 4.77662536438264365890433908527E-1;
+constexpr double a104 = -2.48811461997166764192642586468E0;
+constexpr double a105 = -5.90290826836842996371446475743E-1;
+constexpr double a106 = 2.12300514481811942347288949897E1;
+constexpr double a107 = 1.52792336328824235832596922938E1;
+constexpr double a108 = -3.32882109689848629194453265587E1;
+constexpr double a109 = -2.03312017085086261358222928593E-2;
+constexpr double a111 = -9.3714243008598732571704021658E-1;
+constexpr double a114 = 5.18637242884406370830023853209E0;
+constexpr double a115 = 1.09143734899672957818500254654E0;
+constexpr double a116 = -8.14978701074692612513997267357E0;
+constexpr double a117 = -1.85200656599969598641566180701E1;
+constexpr double a118 = 2.27394870993505042818970056734E1;
+constexpr double a119 = 2.49360555267965238987089396762E0;
+constexpr double a1110 = -3.0467644718982195003823669022E0;
+constexpr double a121 = 2.27331014751653820792359768449E0;
+constexpr double a124 = -1.05344954667372501984066689879E1;
+constexpr double a125 = -2.00087205822486249909675718444E0;
+constexpr double a126 = -1.79589318631187989172765950534E1;
+constexpr double a127 = 2.79488845294199600508499808837E1;
+constexpr double a128 = -2.85899827713502369474065508674E0;
+constexpr double a129 = -8.87285693353062954433549289258E0;
+constexpr double a1210 = 1.23605671757943030647266201528E1;
+constexpr double a1211 = 6.43392746015763530355970484046E-1;
+}  // namespace dop
+
+// Loop state of one integrateDOP call (src/integrator.cpp:93-126).
+struct DopState {
+    double x, xf, facold, posneg;
+    d3 k1;
+    int32_t reject, last;
+    uint32_t nstep;
+};
+
+// ---- exact variant --------------------------------------------------------
+// src/integrator.cpp:114-126: call set-up, including the first RHS eval.
+ST_HD __forceinline__ void dop_begin_exact(const SimParams& P, DopState& D, const d3& y,
+                                                const Seg& S, double t_end) {
+    D.x = S.t0;
+    D.xf = t_end;
+    D.facold = 1.0E-4;
+    D.posneg = sign_d(1.0, t_end - S.t0);
+    D.last = 0;
+    D.reject = 0;
+    D.nstep = 0;
+    D.k1 = rhs_exact(P, D.x, y, S);
+}
+
+#define ST_V3(dst, EX, EY, EZ) do { (dst).x = (EX); (dst).y = (EY); (dst).z = (EZ); } while (0)
+
+// One iteration of the reference's while(1) (src/integrator.cpp:128-247).
+// Returns 0 to continue, 1 when the call returns 1 (segment done), -1 when
+// it returns -1 (nstep > nmax).  h is the caller's step size (tempH).
+ST_HD inline int dop_trial_exact(const SimParams& P, DopState& D, d3& y, double& h,
+                                      const Seg& S, uint32_t& nacc, uint32_t& nrej) {
+    using namespace dop;
+    if (D.nstep > P.nmax) return -1;
+    if ((D.x + 1.01 * h - D.xf) * D.posneg > 0.0) {
+        h = D.xf - D.x;
+        D.last = 1;
+    }
+    if (h < P.hmin && D.last != 1) h = P.hmin;
+    D.nstep++;
+    const double x = D.x;
+    const d3 k1 = D.k1;
+    d3 yy, k2, k3, k4, k5, k6, k7, k8, k9, k10;
+    {
+        const double ha = h * a21;
+        ST_V3(yy, y.x + k1.x * ha, y.y + k1.y * ha, y.z + k1.z * ha);
+    }
+    k2 = rhs_exact(P, x + c2 * h, yy, S);
+    ST_V3(yy, y.x + h * (a31 * k1.x + a32 * k2.x), y.y + h * (a31 * k1.y + a32 * k2.y),
+          y.z + h * (a31 * k1.z + a32 * k2.z));
+    k3 = rhs_exact(P, x + c3 * h, yy, S);
+    ST_V3(yy, y.x + h * (a41 * k1.x + a43 * k3.x), y.y + h * (a41 * k1.y + a43 * k3.y),
+          y.z + h * (a41 * k1.z + a43 * k3.z));
+    k4 = rhs_exact(P, x + c4 * h, yy, S);
+    ST_V3(yy, y.x + h * (a51 * k1.x + a53 * k3.x + a54 * k4.x),
+          y.y + h * (a51 * k1.y + a53 * k3.y + a54 * k4.y),
+          y.z + h * (a51 * k1.z + a53 * k3.z + a54 * k4.z));
+    k5 = rhs_exact(P, x + c5 * h, yy, S);
+    ST_V3(yy, y.x + h * (a61 * k1.x + a64 * k4.x + a65 * k5.x),
+          y.y + h * (a61 * k1.y + a64 * k4.y + a65 * k5.y),
+          y.z + h * (a61 * k1.z + a64 * k4.z + a65 * k5.z));
+    k6 = rhs_exact(P, x + c6 * h, yy, S);
+    ST_V3(yy, y.x + h * (a71 * k1.x + a74 * k4.x + a75 * k5.x + a76 * k6.x),
+          y.y + h * (a71 * k1.y + a74 * k4.y + a75 * k5.y + a76 * k6.y),
+          y.z + h * (a71 * k1.z + a74 * k4.z + a75 * k5.z + a76 * k6.z));
+    k7 = rhs_exact(P, x + c7 * h, yy, S);
+    ST_V3(yy, y.x + h * (a81 * k1.x + a84 * k4.x + a85 * k5.x + a86 * k6.x + a87 * k7.x),
+          y.y + h * (a81 * k1.y + a84 * k4.y + a85 * k5.y + a86 * k6.y + a87 * k7.y),
+          y.z + h * (a81 * k1.z + a84 * k4.z + a85 * k5.z + a86 * k6.z + a87 * k7.z));
+    k8 = rhs_exact(P, x + c8 * h, yy, S);
+    ST_V3(yy, y.x + h * (a91 * k1.x + a94 * k4.x + a95 * k5.x + a96 * k6.x + a97 * k7.x + a98 * k8.x),
+          y.y + h * (a91 * k1.y + a94 * k4.y + a95 * k5.y + a96 * k6.y + a97 * k7.y + a98 * k8.y),
+          y.z + h * (a91 * k1.z + a94 * k4.z + a95 * k5.z + a96 * k6.z + a97 * k7.z + a98 * k8.z));
+    k9 = rhs_exact(P, x + c9 * h, yy, S);
+    ST_V3(yy,
+          y.x + h * (a101 * k1.x + a104 * k4.x + a105 * k5.x + a106 * k6.x + a107 * k7.x + a108 * k8.x + a109 * k9.x),
+          y.y + h * (a101 * k1.y + a104 * k4.y + a105 * k5.y + a106 * k6.y + a107 * k7.y + a108 * k8.y + a109 * k9.y),
+          y.z + h * (a101 * k1.z + a104 * k4.z + a105 * k5.z + a106 * k6.z + a107 * k7.z + a108 * k8.z + a109 * k9.z));
+    k10 = rhs_exact(P, x + c10 * h, yy, S);
+    ST_V3(yy,
+          y.x + h * (a111 * k1.x + a114 * k4.x + a115 * k5.x + a116 * k6.x + a117 * k7.x + a118 * k8.x + a119 * k9.x + a1110 * k10.x),
+          y.y + h * (a111 * k1.y + a114 * k4.y + a115 * k5.y + a116 * k6.y + a117 * k7.y + a118 * k8.y + a119 * k9.y + a1110 * k10.y),
+          y.z + h * (a111 * k1.z + a114 * k4.z + a115 * k5.z + a116 * k6.z + a117 * k7.z + a118 * k8.z + a119 * k9.z + a1110 * k10.z));
+    k2 = rhs_exact(P, x + c11 * h, yy, S);
+    const double xph = x + h;
+    ST_V3(yy,
+          y.x + h * (a121 * k1.x + a124 * k4.x + a125 * k5.x + a126 * k6.x + a127 * k7.x + a128 * k8.x + a129 * k9.x + a1210 * k10.x + a1211 * k2.x),
+          y.y + h * (a121 * k1.y + a124 * k4.y + a125 * k5.y + a126 * k6.y + a127 * k7.y + a128 * k8.y + a129 * k9.y + a1210 * k10.y + a1211 * k2.y),
+          y.z + h * (a121 * k1.z + a124 * k4.z + a125 * k5.z + a126 * k6.z + a127 * k7.z + a128 * k8.z + a129 * k9.z + a1210 * k10.z + a1211 * k2.z));
+    k3 = rhs_exact(P, xph, yy, S);
+    ST_V3(k4, b1 * k1.x + b6 * k6.x + b7 * k7.x + b8 * k8.x + b9 * k9.x + b10 * k10.x + b11 * k2.x + b12 * k3.x,
+          b1 * k1.y + b6 * k6.y + b7 * k7.y + b8 * k8.y + b9 * k9.y + b10 * k10.y + b11 * k2.y + b12 * k3.y,
+          b1 * k1.z + b6 * k6.z + b7 * k7.z + b8 * k8.z + b9 * k9.z + b10 * k10.z + b11 * k2.z + b12 * k3.z);
+    ST_V3(k5, y.x + h * k4.x, y.y + h * k4.y, y.z + h * k4.z);
+    // error estimation (:191-205)
+    d3 sk;
+    ST_V3(sk, P.atol + P.rtol * max_d(fabs(y.x), fabs(k5.x)), P.atol + P.rtol * max_d(fabs(y.y), fabs(k5.y)),
+          P.atol + P.rtol * max_d(fabs(y.z), fabs(k5.z)));
+    double ex = (k4.x - bhh1 * k1.x - bhh2 * k9.x - bhh3 * k3.x) / sk.x;
+    double ey = (k4.y - bhh1 * k1.y - bhh2 * k9.y - bhh3 * k3.y) / sk.y;
+    double ez = (k4.z - bhh1 * k1.z - bhh2 * k9.z - bhh3 * k3.z) / sk.z;
+    const double err2 = 0.0 + (ex * ex + ey * ey + ez * ez);
+    ex = (er1 * k1.x + er6 * k6.x + er7 * k7.x + er8 * k8.x + er9 * k9.x + er10 * k10.x + er11 * k2.x + er12 * k3.x) / sk.x;
+    ey = (er1 * k1.y + er6 * k6.y + er7 * k7.y + er8 * k8.y + er9 * k9.y + er10 * k10.y + er11 * k2.y + er12 * k3.y) / sk.y;
+    ez = (er1 * k1.z + er6 * k6.z + er7 * k7.z + er8 * k8.z + er9 * k9.z + er10 * k10.z + er11 * k2.z + er12 * k3.z) / sk.z;
+    double err = 0.0 + (ex * ex + ey * ey + ez * ez);
+    double deno = err + 0.01 * err2;
+    if (deno <= 0.0) deno = 1.0;
+    err = fabs(h) * err * sqrt(1.0 / (deno * 3.0));
+    // step-size control (:208-213)
+    const double fac11 = pow(err, P.expo1);
+    double fac = P.pow_beta_on ? fac11 / pow(D.facold, P.beta) : fac11 / 1.0;
+    fac = max_d(P.facc2, min_d(P.facc1, fac / P.safe));
+    double hnew = h / fac;
+    if (err <= 1.0 || P.fixed) {
+        D.facold = max_d(err, 1.0E-4);
+        nacc++;
+        D.k1 = rhs_exact(P, xph, k5, S);
+        y = k5;
+        D.x = xph;
+        if (D.last) return 1;
+        if (fabs(hnew) > P.hmax) hnew = D.posneg * P.hmax;
+        if (D.reject) hnew = D.posneg * min_d(fabs(hnew), fabs(h));
+        D.reject = 0;
+    } else {
+        nrej++;
+        hnew = h / min_d(P.facc1, fac11 / P.safe);
+        D.reject = 1;
+        D.last = 0;
+    }
+    if (!P.fixed) h = hnew;
+    return 0;
+}
+
+// ---- fast variant ---------------------------------------------------------
+// Same trial sequence and controller.  Differences, all rounding-level:
+// the RHS comes from the per-segment polynomial (rhs_fast); h is folded
+// into gamma*B so each stage yields K = h*k directly; products contract into
+// FMAs; the six error-norm divisions become multiplies by 1/sk; and with
+// beta == 0 (the reference default) pow(err, 1/8) is three square roots.
+ST_HD __forceinline__ void dop_begin_fast(const SimParams& P, DopState& D, const d3& y,
+                                               const Seg& S, const Poly& q, double t_end) {
+    D.x = S.t0;
+    D.xf = t_end;
+    D.facold = 1.0E-4;
+    D.posneg = sign_d(1.0, t_end - S.t0);
+    D.last = 0;
+    D.reject = 0;
+    D.nstep = 0;
+    D.k1 = rhs_fast(P, D.x, y, S.t0, q);
+}
+
+ST_HD __forceinline__ d3 krhs(const SimParams& P, double t, double tau, const d3& y,
+                                   const Poly& hq, double h) {
+#pragma clang fp contract(fast)
+    d3 w{hq.w0.x + tau * (hq.w1.x + tau * hq.w2.x), hq.w0.y + tau * (hq.w1.y + tau * hq.w2.y),
+         hq.w0.z + tau * (hq.w1.z + tau * hq.w2.z)};
+    if (P.pulse_on) {
+        const double pv = h * (P.gamma * (P.pa * cos(P.pw * t)));
+        if (P.pulse_axis == 0) w.x += pv;
+        else if (P.pulse_axis == 1) w.y += pv;
+        else w.z += pv;
+    }
+    return cross3(y, w);
+}
+
+ST_HD inline int dop_trial_fast(const SimParams& P, DopState& D, d3& y, double& h,
+                                     const Seg& S, const Poly& q, uint32_t& nacc, uint32_t& nrej) {
+#pragma clang fp contract(fast)
+    using namespace dop;
+    if (D.nstep > P.nmax) return -1;
+    if ((D.x + 1.01 * h - D.xf) * D.posneg > 0.0) {
+        h = D.xf - D.x;
+        D.last = 1;
+    }
+    if (h < P.hmin && D.last != 1) h = P.hmin;
+    D.nstep++;
+    const double x = D.x;
+    const double tx = x - S.t0;
+    const Poly hq{d3{h * q.w0.x, h * q.w0.y, h * q.w0.z}, d3{h * q.w1.x, h * q.w1.y, h * q.w1.z},
+                  d3{h * q.w2.x, h * q.w2.y, h * q.w2.z}};
+    const d3 K1{h * D.k1.x, h * D.k1.y, h * D.k1.z};
+#define KR(C, YY) krhs(P, x + (C) * h, tx + (C) * h, (YY), hq, h)
+    d3 yy, K2, K3, K4, K5, K6, K7, K8, K9, K10, K11, K12;
+    ST_V3(yy, y.x + a21 * K1.x, y.y + a21 * K1.y, y.z + a21 * K1.z);
+    K2 = KR(c2, yy);
+    ST_V3(yy, y.x + (a31 * K1.x + a32 * K2.x), y.y + (a31 * K1.y + a32 * K2.y),
+          y.z + (a31 * K1.z + a32 * K2.z));
+    K3 = KR(c3, yy);
+    ST_V3(yy, y.x + (a41 * K1.x + a43 * K3.x), y.y + (a41 * K1.y + a43 * K3.y),
+          y.z + (a41 * K1.z + a43 * K3.z));
+    K4 = KR(c4, yy);
+    ST_V3(yy, y.x + (a51 * K1.x + a53 * K3.x + a54 * K4.x), y.y + (a51 * K1.y + a53 * K3.y + a54 * K4.y),
+          y.z + (a51 * K1.z + a53 * K3.z + a54 * K4.z));
+    K5 = KR(c5, yy);
+    ST_V3(yy, y.x + (a61 * K1.x + a64 * K4.x + a65 * K5.x), y.y + (a61 * K1.y + a64 * K4.y + a65 * K5.y),
+          y.z + (a61 * K1.z + a64 * K4.z + a65 * K5.z));
+    K6 = KR(c6, yy);
+    ST_V3(yy, y.x + (a71 * K1.x + a74 * K4.x + a75 * K5.x + a76 * K6.x),
+          y.y + (a71 * K1.y + a74 * K4.y + a75 * K5.y + a76 * K6.y),
+          y.z + (a71 * K1.z + a74 * K4.z + a75 * K5.z + a76 * K6.z));
+    K7 = KR(c7, yy);
+    ST_V3(yy, y.x + (a81 * K1.x + a84 * K4.x + a85 * K5.x + a86 * K6.x + a87 * K7.x),
+          y.y + (a81 * K1.y + a84 * K4.y + a85 * K5.y + a86 * K6.y + a87 * K7.y),
+          y.z + (a81 * K1.z + a84 * K4.z + a85 * K5.z + a86 * K6.z + a87 * K7.z));
+    K8 = KR(c8, yy);
+    ST_V3(yy, y.x + (a91 * K1.x + a94 * K4.x + a95 * K5.x + a96 * K6.x + a97 * K7.x + a98 * K8.x),
+          y.y + (a91 * K1.y + a94 * K4.y + a95 * K5.y + a96 * K6.y + a97 * K7.y + a98 * K8.y),
+          y.z + (a91 * K1.z + a94 * K4.z + a95 * K5.z + a96 * K6.z + a97 * K7.z + a98 * K8.z));
+    K9 = KR(c9, yy);
+    ST_V3(yy, y.x + (a101 * K1.x + a104 * K4.x + a105 * K5.x + a106 * K6.x + a107 * K7.x + a108 * K8.x + a109 * K9.x),
+          y.y + (a101 * K1.y + a104 * K4.y + a105 * K5.y + a106 * K6.y + a107 * K7.y + a108 * K8.y + a109 * K9.y),
+          y.z + (a101 * K1.z + a104 * K4.z + a105 * K5.z + a106 * K6.z + a107 * K7.z + a108 * K8.z + a109 * K9.z));
+    K10 = KR(c10, yy);
+    ST_V3(yy,
+          y.x + (a111 * K1.x + a114 * K4.x + a115 * K5.x + a116 * K6.x + a117 * K7.x + a118 * K8.x + a119 * K9.x + a1110 * K10.x),
+          y.y + (a111 * K1.y + a114 * K4.y + a115 * K5.y + a116 * K6.y + a117 * K7.y + a118 * K8.y + a119 * K9.y + a1110 * K10.y),
+          y.z + (a111 * K1.z + a114 * K4.z + a115 * K5.z + a116 * K6.z + a117 * K7.z + a118 * K8.z + a119 * K9.z + a1110 * K10.z));
+    K11 = KR(c11, yy);
+    const double xph = x + h;
+    ST_V3(yy,
+          y.x + (a121 * K1.x + a124 * K4.x + a125 * K5.x + a126 * K6.x + a127 * K7.x + a128 * K8.x + a129 * K9.x + a1210 * K10.x + a1211 * K11.x),
+          y.y + (a121 * K1.y + a124 * K4.y + a125 * K5.y + a126 * K6.y + a127 * K7.y + a128 * K8.y + a129 * K9.y + a1210 * K10.y + a1211 * K11.y),
+          y.z + (a121 * K1.z + a124 * K4.z + a125 * K5.z + a126 * K6.z + a127 * K7.z + a128 * K8.z + a129 * K9.z + a1210 * K10.z + a1211 * K11.z));
+    K12 = krhs(P, xph, tx + h, yy, hq, h);
+#undef KR
+    d3 bs;
+    ST_V3(bs, b1 * K1.x + b6 * K6.x + b7 * K7.x + b8 * K8.x + b9 * K9.x + b10 * K10.x + b11 * K11.x + b12 * K12.x,
+          b1 * K1.y + b6 * K6.y + b7 * K7.y + b8 * K8.y + b9 * K9.y + b10 * K10.y + b11 * K11.y + b12 * K12.y,
+          b1 * K1.z + b6 * K6.z + b7 * K7.z + b8 * K8.z + b9 * K9.z + b10 * K10.z + b11 * K11.z + b12 * K12.z);
+    d3 y1;
+    ST_V3(y1, y.x + bs.x, y.y + bs.y, y.z + bs.z);
+    // error norm: with K = h*k the reference's |h| * err / sqrt(n*deno)
+    // becomes errK / sqrt(n*denoK)
+    const double isx = 1.0 / (P.atol + P.rtol * max_d(fabs(y.x), fabs(y1.x)));
+    const double isy = 1.0 / (P.atol + P.rtol * max_d(fabs(y.y), fabs(y1.y)));
+    const double isz = 1.0 / (P.atol + P.rtol * max_d(fabs(y.z), fabs(y1.z)));
+    double ex = (bs.x - bhh1 * K1.x - bhh2 * K9.x - bhh3 * K12.x) * isx;
+    double ey = (bs.y - bhh1 * K1.y - bhh2 * K9.y - bhh3 * K12.y) * isy;
+    double ez = (bs.z - bhh1 * K1.z - bhh2 * K9.z - bhh3 * K12.z) * isz;
+    const double err2 = ex * ex + ey * ey + ez * ez;
+    ex = (er1 * K1.x + er6 * K6.x + er7 * K7.x + er8 * K8.x + er9 * K9.x + er10 * K10.x + er11 * K11.x + er12 * K12.x) * isx;
+    ey = (er1 * K1.y + er6 * K6.y + er7 * K7.y + er8 * K8.y + er9 * K9.y + er10 * K10.y + er11 * K11.y + er12 * K12.y) * isy;
+    ez = (er1 * K1.z + er6 * K6.z + er7 * K7.z + er8 * K8.z + er9 * K9.z + er10 * K10.z + er11 * K11.z + er12 * K12.z) * isz;
+    const double e1 = ex * ex + ey * ey + ez * ez;
+    const double deno = e1 + 0.01 * err2;
+    const double err = (deno <= 0.0) ? 0.0 : e1 / sqrt(3.0 * deno);
+    double fac11;
+    if (!P.pow_beta_on && P.expo1 == 0.125)
+        fac11 = sqrt(sqrt(sqrt(err)));
+    else
+        fac11 = pow(err, P.expo1);
+    double fac = P.pow_beta_on ? fac11 / pow(D.facold, P.beta) : fac11;
+    fac = max_d(P.facc2, min_d(P.facc1, fac / P.safe));
+    double hnew = h / fac;
+    if (err <= 1.0 || P.fixed) {
+        D.facold = max_d(err, 1.0E-4);
+        nacc++;
+        D.k1 = rhs_fast(P, xph, y1, S.t0, q);
+        y = y1;
+        D.x = xph;
+        if (D.last) return 1;
+        if (fabs(hnew) > P.hmax) hnew = D.posneg * P.hmax;
+        if (D.reject) hnew = D.posneg * min_d(fabs(hnew), fabs(h));
+        D.reject = 0;
+    } else {
+        nrej++;
+        hnew = h / min_d(P.facc1, fac11 / P.safe);
+        D.reject = 1;
+        D.last = 0;
+    }
+    if (!P.fixed) h = hnew;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// RK45 (src/integrator.cpp:251-337, tableau include/coeff.h:178-218)
+// ---------------------------------------------------------------------------
+namespace rk {
+constexpr double A2 = 2.0 / 9.0, A3 = 1.0 / 3.0, A4 = 3.0 / 4.0, A5 = 1.0, A6 = 5.0 / 6.0;
+constexpr double B21 = 2.0 / 9.0;
+constexpr double B31 = 1.0 / 12.0, B32 = 1.0 / 4.0;
+constexpr double B41 = 69.0 / 128.0, B42 = -243.0 / 128.0, B43 = 135.0 / 64.0;
+constexpr double B51 = -17.0 / 12.0, B52 = 27.0 / 4.0, B53 = -27.0 / 5.0, B54 = 16.0 / 15.0;
+constexpr double B61 = 65.0 / 432.0, B62 = -5.0 / 16.0, B63 = 13.0 / 16.0, B64 = 4.0 / 27.0,
+                 B65 = 5.0 / 144.0;
+constexpr double CH1 = 47.0 / 450.0, CH2 = 0.0, CH3 = 12.0 / 25.0, CH4 = 32.0 / 225.0,
+                 CH5 = 1.0 / 30.0, CH6 = 6.0 / 25.0;
+constexpr double CT1 = 1.0 / 150.0, CT2 = 0.0, CT3 = -3.0 / 100.0, CT4 = 16.0 / 75.0,
+                 CT5 = 1.0 / 20.0, CT6 = -6.0 / 25.0;
+}  // namespace rk
+
+// Loop state of one integrateRK45 call (src/integrator.cpp:255-266).  yy1
+// persists across trials: the first trial's k1 is evaluated at the zero
+// vector and later trials' k1 at the previous trial's sixth-stage input
+// (:298), exactly as the reference does.
+struct RkState {
+    double t, tf, lastH, prev_ratio;
+    d3 yy1;
+    int32_t nstep;
+    bool stop;
+};
+
+ST_HD __forceinline__ void rk_begin(RkState& R, const Seg& S, double t_end, double h) {
+    R.t = S.t0;
+    R.tf = t_end;
+    R.stop = false;
+    R.nstep = 0;
+    R.lastH = h;
+    R.yy1 = d3{0.0, 0.0, 0.0};
+    R.prev_ratio = 1.0;
+}
+
+// One iteration of the reference's while(1) (:267-335).  Returns 0 to
+// continue, 1 when the call returns 0 (done), -1 / -2 for the reference's
+// error returns.
+template <int VAR>
+ST_HD inline int rk_trial(const SimParams& P, RkState& R, d3& y, double& h, const Seg& S,
+                               const Poly& q, uint32_t& nacc, uint32_t& nrej) {
+    using namespace rk;
+    auto F = [&](double tt, const d3& yy) -> d3 {
+        if (VAR == VAR_EXACT) return rhs_exact(P, tt, yy, S);
+        return rhs_fast(P, tt, yy, S.t0, q);
+    };
+    R.nstep++;
+    if ((uint32_t)R.nstep > P.nmax) return -2;
+    const double endDt = R.tf - R.t;
+    R.lastH = h;
+    if (P.fixed) {
+        h = P.h0;
+    } else {
+        if (h > P.hmax) {
+            h = std_min(h, P.hmax);
+        } else if (h < P.hmin) {
+            return -1;
+        }
+    }
+    if (h >= endDt) {
+        h = endDt;
+        R.stop = true;
+    } else {
+        R.lastH = h;
+        R.stop = false;
+    }
+    const double t = R.t;
+    d3 yy1 = R.yy1;
+    const d3 k1 = F(t, yy1);
+    {
+        const double c = h * B21;
+        ST_V3(yy1, y.x + c * k1.x, y.y + c * k1.y, y.z + c * k1.z);
+    }
+    const d3 k2 = F(t + A2 * h, yy1);
+    {
+        const double c1 = h * B31, c2_ = h * B32;
+        ST_V3(yy1, y.x + c1 * k1.x + c2_ * k2.x, y.y + c1 * k1.y + c2_ * k2.y, y.z + c1 * k1.z + c2_ * k2.z);
+    }
+    const d3 k3 = F(t + A3 * h, yy1);
+    {
+        const double c1 = h * B41, c2_ = h * B42, c3_ = h * B43;
+        ST_V3(yy1, y.x + c1 * k1.x + c2_ * k2.x + c3_ * k3.x, y.y + c1 * k1.y + c2_ * k2.y + c3_ * k3.y,
+              y.z + c1 * k1.z + c2_ * k2.z + c3_ * k3.z);
+    }
+    const d3 k4 = F(t + A4 * h, yy1);
+    {
+        const double c1 = h * B51, c2_ = h * B52, c3_ = h * B53, c4_ = h * B54;
+        ST_V3(yy1, y.x + c1 * k1.x + c2_ * k2.x + c3_ * k3.x + c4_ * k4.x,
+              y.y + c1 * k1.y + c2_ * k2.y + c3_ * k3.y + c4_ * k4.y,
+              y.z + c1 * k1.z + c2_ * k2.z + c3_ * k3.z + c4_ * k4.z);
+    }
+    const d3 k5 = F(t + A5 * h, yy1);
+    {
+        const double c1 = h * B61, c2_ = h * B62, c3_ = h * B63, c4_ = h * B64, c5_ = h * B65;
+        ST_V3(yy1, y.x + c1 * k1.x + c2_ * k2.x + c3_ * k3.x + c4_ * k4.x + c5_ * k5.x,
+              y.y + c1 * k1.y + c2_ * k2.y + c3_ * k3.y + c4_ * k4.y + c5_ * k5.y,
+              y.z + c1 * k1.z + c2_ * k2.z + c3_ * k3.z + c4_ * k4.z + c5_ * k5.z);
+    }
+    const d3 k6 = F(t + A6 * h, yy1);
+    R.yy1 = yy1;
+    d3 ws, te;
+    ST_V3(ws, y.x + h * (k1.x * CH1 + k2.x * CH2 + k3.x * CH3 + k4.x * CH4 + k5.x * CH5 + k6.x * CH6),
+          y.y + h * (k1.y * CH1 + k2.y * CH2 + k3.y * CH3 + k4.y * CH4 + k5.y * CH5 + k6.y * CH6),
+          y.z + h * (k1.z * CH1 + k2.z * CH2 + k3.z * CH3 + k4.z * CH4 + k5.z * CH5 + k6.z * CH6));
+    ST_V3(te, h * (CT1 * k1.x + CT2 * k2.x + CT3 * k3.x + CT4 * k4.x + CT5 * k5.x + CT6 * k6.x),
+          h * (CT1 * k1.y + CT2 * k2.y + CT3 * k3.y + CT4 * k4.y + CT5 * k5.y + CT6 * k6.y),
+          h * (CT1 * k1.z + CT2 * k2.z + CT3 * k3.z + CT4 * k4.z + CT5 * k5.z + CT6 * k6.z));
+    double error = len3(te);
+    error = std_max(error, 1.0E-16);
+    const double tol = P.rtol;
+    const double ratio = tol / error;
+    double q_ = pow(ratio, 0.7 / 7.0) * pow(R.prev_ratio, -0.4 / 7.0);
+    q_ = std_min(q_, 4.0);
+    if ((q_ > 0.81 && error < 2 * tol) || P.fixed) {
+        nacc++;
+        y = ws;
+        R.t += h;
+        if (R.stop) {
+            h = R.lastH;
+            return 1;
+        }
+    } else {
+        nrej++;
+        if (R.stop) R.stop = false;
+    }
+    if (!P.fixed) h = h * q_;
+    R.prev_ratio = ratio;
+    return 0;
+}
+
+#undef ST_V3
+
+}  // namespace st

@@ -1,0 +1,135 @@
+"""ctypes binding of the gfx950 library ``libspintrack.so`` (include/spintrack.h).
+
+The library is built in-tree by ``spin-tracking_amd/Makefile``.  Loading fails
+loudly when it is missing: there is no CPU fallback in the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_ROOT, "libspintrack.so")
+
+ST_OK = 0
+ST_E_ARG = -1
+ST_E_HIP = -2
+ST_E_UNSUPPORTED = -3
+ST_E_STATE = -4
+
+VARIANT_EXACT = 0
+VARIANT_FAST = 1
+
+
+class Coords(C.Structure):
+    """include/double3.h:17-21"""
+    _fields_ = [("x", C.c_double), ("y", C.c_double), ("z", C.c_double)]
+
+    def __iter__(self):
+        return iter((self.x, self.y, self.z))
+
+    def __repr__(self):
+        return f"({self.x!r}, {self.y!r}, {self.z!r})"
+
+
+_DOUBLE_FIELDS = ["m", "t0", "tf", "rtol", "atol", "beta", "uround", "safe", "fac1", "fac2",
+                  "hmax", "hmin", "h", "T", "sqrtKT_m", "tc", "gamma", "V", "a", "w",
+                  "swapStepSize", "maxPosStep", "ioutInt", "diffuse"]
+_COORD_FIELDS = ["B0", "E", "L", "yi", "Gx", "Gy", "Gz"]
+
+
+class Options(C.Structure):
+    """The reference's 440-byte ``options`` block (include/options.h:10-61)."""
+    _fields_ = ([(n, Coords) for n in _COORD_FIELDS]
+                + [(n, C.c_double) for n in _DOUBLE_FIELDS]
+                + [("nmax", C.c_uint32), ("seed", C.c_uint32), ("integratorType", C.c_int32),
+                   ("numParticles", C.c_int32), ("numPerGPUBlock", C.c_int32), ("iout", C.c_int32),
+                   ("dist", C.c_char), ("gas_coll", C.c_uint8), ("gravity", C.c_uint8),
+                   ("fixedStepSize", C.c_uint8), ("keepStepSize", C.c_uint8),
+                   ("noiseAmplitudes", Coords), ("noiseFrequencies", Coords)])
+
+
+assert C.sizeof(Options) == 440, C.sizeof(Options)
+
+
+class Ext(C.Structure):
+    _fields_ = [("pulse_axis", C.c_int32), ("variant", C.c_int32), ("block_size", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+class Rng(C.Structure):
+    """include/particle.h:52-59"""
+    _fields_ = [("x", C.c_uint64), ("y", C.c_uint64), ("z", C.c_uint64), ("w", C.c_uint64),
+                ("spare", C.c_double), ("hasSpare", C.c_uint8), ("pad_", C.c_uint8 * 7)]
+
+
+assert C.sizeof(Rng) == 48
+
+
+class ParticleState(C.Structure):
+    _fields_ = [("S", Coords), ("v", Coords), ("v_old", Coords), ("pos", Coords),
+                ("pos_old", Coords), ("t", C.c_double), ("t_old", C.c_double),
+                ("tf", C.c_double), ("next_gas", C.c_double), ("h", C.c_double),
+                ("rng", Rng), ("n_bounce", C.c_uint64), ("n_coll", C.c_uint64),
+                ("n_steps", C.c_uint64), ("failure", C.c_int32), ("part_id", C.c_uint32),
+                ("stop", C.c_uint8), ("coll_type", C.c_uint8), ("wall_hit", C.c_uint8),
+                ("pad_", C.c_uint8 * 5)]
+
+
+# every symbol include/spintrack.h declares, with its ctypes signature
+_vp, _d, _i32, _u32, _sz = C.c_void_p, C.c_double, C.c_int32, C.c_uint32, C.c_size_t
+SIGNATURES = {
+    "st_version": (C.c_char_p, []),
+    "st_last_error": (C.c_char_p, [_vp]),
+    "st_create": (C.c_int, [C.POINTER(Options), C.POINTER(Ext), C.POINTER(_vp)]),
+    "st_destroy": (None, [_vp]),
+    "st_set_stream": (C.c_int, [_vp, _vp]),
+    "st_init": (C.c_int, [_vp]),
+    "st_init_subset": (C.c_int, [_vp, _u32, _u32]),
+    "st_advance_to": (C.c_int, [_vp, _d]),
+    "st_snapshot": (C.c_int, [_vp] + [_vp] * 8),
+    "st_snapshot_block_bytes": (_sz, [_vp]),
+    "st_snapshot_block_device": (C.c_int, [_vp, _vp]),
+    "st_run": (C.c_int, [_vp, _i32, _vp, _vp]),
+    "st_set_stop_times": (C.c_int, [_vp, _vp]),
+    "st_get_state": (C.c_int, [_vp, _vp]),
+    "st_set_state": (C.c_int, [_vp, _vp]),
+    "st_work_counters": (C.c_int, [_vp, _vp]),
+    "st_particle_counters": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "st_synchronize": (C.c_int, [_vp]),
+    "st_set_trace": (C.c_int, [_vp, C.c_int64, _u32]),
+    "st_get_trace": (C.c_int, [_vp, _vp, _u32, C.POINTER(_u32)]),
+}
+
+TRACE_DOUBLES = 20
+TRACE_FIELDS = ["tag", "t_old", "t", "px", "py", "pz", "vx", "vy", "vz", "sx", "sy", "sz", "h",
+                "accepted", "rejected", "coll_type", "wall_hit", "status", "n_bounce", "n_coll"]
+
+_lib = None
+
+
+class SpinTrackError(RuntimeError):
+    pass
+
+
+def load() -> C.CDLL:
+    """Loads libspintrack.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SpinTrackError(
+                f"{LIB_PATH} is missing: build it with `make -C spin-tracking_amd` "
+                "(or __graft_entry__.build()); there is no CPU fallback")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int, handle=None) -> None:
+    if rc != ST_OK:
+        msg = load().st_last_error(handle)
+        raise SpinTrackError(f"spintrack error {rc}: {msg.decode() if msg else ''}")

@@ -1,0 +1,171 @@
+"""Host-side mirror of the reference's ``particle`` class (include/particle.h:87-314).
+
+    p = Particle(opt)            # particle p(opt)           -> st_create
+    p.initParticles()            # p.initParticles()         -> st_init
+    p.runSimulation(next_t)      # p.runSimulation(nextTOut) -> st_advance_to
+    p.outputData(logger)         # p.outputData(log)         -> st_snapshot + writeSnapshot
+
+All particle state stays in HBM; only the snapshot / state copy-outs move
+data to the host.  ``run(targets, blocks)`` is the persistent multi-interval
+form (st_run), which produces the same per-particle results as calling
+``runSimulation`` for every target in turn.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import Ext, Options, ParticleState, check
+
+COORDS = np.dtype([("x", "<f8"), ("y", "<f8"), ("z", "<f8")])
+SNAPSHOT_FIELDS = ("t", "pos", "v", "S", "failure", "n_coll", "n_bounce", "n_steps")
+
+STATE_DTYPE = np.dtype({
+    "names": ["S", "v", "v_old", "pos", "pos_old", "t", "t_old", "tf", "next_gas", "h",
+              "rng_x", "rng_y", "rng_z", "rng_w", "spare", "hasSpare",
+              "n_bounce", "n_coll", "n_steps", "failure", "part_id", "stop", "coll_type",
+              "wall_hit"],
+    "formats": [COORDS] * 5 + ["<f8"] * 5 + ["<u8"] * 4 + ["<f8", "u1"] + ["<u8"] * 3
+               + ["<i4", "<u4", "u1", "u1", "u1"],
+    "offsets": [0, 24, 48, 72, 96, 120, 128, 136, 144, 152, 160, 168, 176, 184, 192, 200,
+                208, 216, 224, 232, 236, 240, 241, 242],
+    "itemsize": C.sizeof(ParticleState),
+})
+
+
+def snapshot_block_dtype(n: int) -> np.dtype:
+    """One snapshot block as BinaryLogger::writeSnapshot lays it out
+    (src/logger.cpp:14-25): eight consecutive arrays of n entries."""
+    return np.dtype({
+        "names": list(SNAPSHOT_FIELDS),
+        "formats": [("<f8", (n,)), (COORDS, (n,)), (COORDS, (n,)), (COORDS, (n,)),
+                    ("<i4", (n,)), ("<u8", (n,)), ("<u8", (n,)), ("<u8", (n,))],
+        "offsets": [0, 8 * n, 32 * n, 56 * n, 80 * n, 84 * n, 92 * n, 100 * n],
+        "itemsize": 108 * n,
+    })
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+class Particle:
+    """GPU-resident ensemble of ``opt.numParticles`` particles."""
+
+    def __init__(self, opt: Options, pulse_axis: int = 2, variant: int = L.VARIANT_EXACT,
+                 block_size: int = 0):
+        self.lib = L.load()
+        self.opt = Options.from_buffer_copy(bytes(opt))
+        self.n = int(opt.numParticles)
+        self.ext = Ext(int(pulse_axis), int(variant), int(block_size), 0)
+        h = C.c_void_p()
+        check(self.lib.st_create(C.byref(self.opt), C.byref(self.ext), C.byref(h)), None)
+        self.h = h
+
+    # -- lifetime -----------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.st_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- the reference's interface -----------------------------------------
+    def initParticles(self, first: int = 0, stride: int = 1) -> None:
+        """include/particle.h:241-252 / src/particle.cpp:638-710."""
+        check(self.lib.st_init_subset(self.h, first, stride), self.h)
+
+    def runSimulation(self, next_t: float) -> None:
+        """src/particle.cpp:865-876: advance every particle to next_t."""
+        check(self.lib.st_advance_to(self.h, float(next_t)), self.h)
+
+    def snapshot(self) -> dict:
+        """The eight arrays of one snapshot, in the reference's types."""
+        n = self.n
+        out = dict(t=np.empty(n), pos=np.empty(n, COORDS), v=np.empty(n, COORDS),
+                   S=np.empty(n, COORDS), failure=np.empty(n, np.int32),
+                   n_coll=np.empty(n, np.uint64), n_bounce=np.empty(n, np.uint64),
+                   n_steps=np.empty(n, np.uint64))
+        check(self.lib.st_snapshot(self.h, *[_ptr(out[k]) for k in SNAPSHOT_FIELDS]), self.h)
+        return out
+
+    def outputData(self, logger) -> None:
+        """src/particle.cpp:852-862."""
+        s = self.snapshot()
+        logger.writeSnapshot(*[s[k] for k in SNAPSHOT_FIELDS])
+
+    # -- extensions ---------------------------------------------------------
+    def run(self, targets, blocks_dev_ptr: int = 0) -> None:
+        """Persistent multi-interval run (st_run)."""
+        t = np.ascontiguousarray(targets, dtype=np.float64)
+        check(self.lib.st_run(self.h, len(t), _ptr(t), blocks_dev_ptr or None), self.h)
+
+    def set_stop_times(self, stop_at) -> None:
+        if stop_at is None:
+            check(self.lib.st_set_stop_times(self.h, None), self.h)
+            return
+        s = np.ascontiguousarray(stop_at, dtype=np.float64)
+        assert s.shape == (self.n,)
+        check(self.lib.st_set_stop_times(self.h, _ptr(s)), self.h)
+
+    def block_bytes(self) -> int:
+        return int(self.lib.st_snapshot_block_bytes(self.h))
+
+    def snapshot_block_device(self, dst_ptr: int) -> None:
+        check(self.lib.st_snapshot_block_device(self.h, dst_ptr), self.h)
+
+    def get_state(self) -> np.ndarray:
+        out = np.zeros(self.n, STATE_DTYPE)
+        check(self.lib.st_get_state(self.h, _ptr(out)), self.h)
+        return out
+
+    def set_state(self, state: np.ndarray) -> None:
+        s = np.ascontiguousarray(state, dtype=STATE_DTYPE)
+        assert s.shape == (self.n,)
+        check(self.lib.st_set_state(self.h, _ptr(s)), self.h)
+
+    def work_counters(self) -> dict:
+        w = np.zeros(4, np.uint64)
+        check(self.lib.st_work_counters(self.h, _ptr(w)), self.h)
+        return dict(accepted=int(w[0]), rejected=int(w[1]), segments=int(w[2]), trips=int(w[3]))
+
+    def particle_counters(self) -> dict:
+        a, r, s = (np.zeros(self.n, np.uint64) for _ in range(3))
+        check(self.lib.st_particle_counters(self.h, _ptr(a), _ptr(r), _ptr(s)), self.h)
+        return dict(accepted=a, rejected=r, segments=s)
+
+    def set_trace(self, particle: int, capacity: int = 100000) -> None:
+        """Record every integrated segment of one particle (debug tracing)."""
+        check(self.lib.st_set_trace(self.h, int(particle), int(capacity)), self.h)
+        self._trace_cap = int(capacity)
+
+    def get_trace(self) -> np.ndarray:
+        """(records, 20) array; columns are spintrack._lib.TRACE_FIELDS."""
+        cap = getattr(self, "_trace_cap", 0)
+        out = np.zeros((max(cap, 1), L.TRACE_DOUBLES))
+        n = C.c_uint32()
+        check(self.lib.st_get_trace(self.h, _ptr(out), cap, C.byref(n)), self.h)
+        return out[: n.value]
+
+    def set_stream(self, stream_ptr: int) -> None:
+        check(self.lib.st_set_stream(self.h, stream_ptr or None), self.h)
+
+    def synchronize(self) -> None:
+        check(self.lib.st_synchronize(self.h), self.h)
+
+
+def xyz(a: np.ndarray) -> np.ndarray:
+    """COORDS structured array -> (n, 3) float64."""
+    return np.stack([a["x"], a["y"], a["z"]], axis=-1)
