@@ -52,21 +52,17 @@ def algorithmic_flops(opt, w):
     return fa * w["accepted"] + FLOPS_REJ * w["rejected"] + FLOPS_EVENT * w["segments"]
 
 
-def run_gpu(opt, args, targets, persistent, variant):
-    import torch
+def run_gpu(opt, args, targets, persistent, variant, engine):
+    """Seeds an ensemble in HBM, then times the K output intervals with HIP
+    events recorded on the stream the library launches on."""
     import spintrack as st
 
-    dev = torch.device("cuda:0")
-    # A dedicated (default-priority) stream: the library launches on it and
-    # the HIP events below are recorded on it, so they bracket the kernels.
-    stream = torch.cuda.Stream(dev)
-    assert stream.cuda_stream != 0
+    kw = dict(variant=variant, engine=engine, seg_buffer_mb=args.seg_buffer_mb)
     # warm-up on its own ensemble (first-launch costs, clocks)
     if args.warmup > 0:
         wopt = type(opt).from_buffer_copy(bytes(opt))
         wopt.seed = opt.seed + 7919
-        with st.Particle(wopt, variant=variant) as w:
-            w.set_stream(stream.cuda_stream)
+        with st.Particle(wopt, **kw) as w:
             w.initParticles()
             wt = targets[: args.warmup]
             if persistent:
@@ -74,30 +70,30 @@ def run_gpu(opt, args, targets, persistent, variant):
             else:
                 for t in wt:
                     w.runSimulation(t)
-            torch.cuda.synchronize()
-    p = st.Particle(opt, variant=variant)
-    p.set_stream(stream.cuda_stream)
+            w.synchronize()
+    p = st.Particle(opt, **kw)
     p.initParticles()
     bb = p.block_bytes()
-    blocks = torch.empty(len(targets) * bb, dtype=torch.uint8, device=dev)
-    torch.cuda.synchronize()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    blocks = st._lib.DeviceBuffer(len(targets) * bb)
+    p.synchronize()
+    if engine == st._lib.ENGINE_PIPELINE:
+        p.kernel_timing(True)
     t0 = time.perf_counter()
-    ev0.record(stream)
+    p.timer_start()
     if persistent:
-        p.run(targets, blocks.data_ptr())
+        p.run(targets, blocks.ptr)
     else:
         for k, t in enumerate(targets):
             p.runSimulation(t)
-            p.snapshot_block_device(blocks.data_ptr() + k * bb)
-    ev1.record(stream)
-    torch.cuda.synchronize()
+            p.snapshot_block_device(blocks.ptr + k * bb)
+    kernel_ms = p.timer_stop()
     wall = time.perf_counter() - t0
-    kernel_ms = ev0.elapsed_time(ev1)
+    kt = p.kernel_times() if engine == st._lib.ENGINE_PIPELINE else None
     w = p.work_counters()
     fails = int((p.snapshot()["failure"] != 0).sum())
+    blocks.free()
     p.close()
-    return dict(kernel_ms=kernel_ms, wall_s=wall, work=w, failures=fails)
+    return dict(kernel_ms=kernel_ms, wall_s=wall, work=w, failures=fails, ktimes=kt)
 
 
 def cpu_baseline(opt, n_total, sample_particles, sample_intervals):
@@ -139,6 +135,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the per-interval (plain) run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--cpu-sample", type=int, default=8192, help="particles in the CPU sample")
+    ap.add_argument("--engine", choices=["pipeline", "fused"], default="pipeline")
+    ap.add_argument("--seg-buffer-mb", type=int, default=0, help="pipeline staging budget (0 = library default)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -152,20 +150,38 @@ def main():
     steps = max(1, min(args.steps, len(all_targets)))
     targets = all_targets[:steps]
     variant = st.VARIANT_FAST if args.variant == "fast" else st.VARIANT_EXACT
-    print(f"[bench] {args.particles} He-3, {steps} intervals of {opt.ioutInt} s, variant={args.variant}",
-          file=sys.stderr, flush=True)
+    engine = st._lib.ENGINE_PIPELINE if args.engine == "pipeline" else st._lib.ENGINE_FUSED
+    print(f"[bench] {args.particles} He-3, {steps} intervals of {opt.ioutInt} s, variant={args.variant}, "
+          f"engine={args.engine}", file=sys.stderr, flush=True)
 
-    res = run_gpu(opt, args, targets, True, variant)
+    res = run_gpu(opt, args, targets, True, variant, engine)
     psec = args.particles * (targets[-1] - opt.t0)
     value = psec / (res["kernel_ms"] / 1e3)
-    flops = algorithmic_flops(opt, res["work"])
-    achieved_tf = flops / (res["kernel_ms"] / 1e3) / 1e12
-    print(f"[bench] persistent: {res['kernel_ms']:.1f} ms, {value:.4g} particle-s/s, "
-          f"work={res['work']}", file=sys.stderr, flush=True)
+    w = res["work"]
+    flops = algorithmic_flops(opt, w)
+    kt = res["ktimes"]
+    if kt:
+        # dominant kernel: the spin kernel; its algorithmic work is the
+        # integrator trial steps (the per-event work runs in k_kin)
+        fa = FLOPS_ACC_GRAV if opt.gravity else FLOPS_ACC_NOGRAV
+        spin_flops = (FLOPS_RK45_TRIAL * (w["accepted"] + w["rejected"]) if opt.integratorType == 1
+                      else fa * w["accepted"] + FLOPS_REJ * w["rejected"])
+        per_launch_ms = kt["spin_ms"] / max(1, kt["spin_launches"])
+        achieved_tf = spin_flops / (kt["spin_ms"] / 1e3) / 1e12
+        kernel = f"k_spin<{args.variant}, {'RK45' if opt.integratorType == 1 else 'DOP853'}>"
+        roof_extra = {"launches": kt["spin_launches"], "avg_launch_ms": per_launch_ms,
+                      "kernel_ms": kt["spin_ms"], "kin_kernel_ms": kt["kin_ms"],
+                      "kin_launches": kt["kin_launches"], "whole_run_TFLOPs": flops / (res["kernel_ms"] / 1e3) / 1e12}
+    else:
+        achieved_tf = flops / (res["kernel_ms"] / 1e3) / 1e12
+        kernel = f"k_engine<{args.variant}, persistent>"
+        roof_extra = {"launches": 1, "avg_launch_ms": res["kernel_ms"]}
+    print(f"[bench] persistent: {res['kernel_ms']:.1f} ms, {value:.4g} particle-s/s, kernels={kt}, "
+          f"work={w}", file=sys.stderr, flush=True)
 
     plain = None
     if not args.no_plain:
-        r2 = run_gpu(opt, args, targets, False, variant)
+        r2 = run_gpu(opt, args, targets, False, variant, engine)
         plain = psec / (r2["kernel_ms"] / 1e3)
         print(f"[bench] plain: {r2['kernel_ms']:.1f} ms, {plain:.4g} particle-s/s", file=sys.stderr, flush=True)
 
@@ -191,12 +207,12 @@ def main():
                                "DOP853 rtol=atol=1e-12, output every 0.5 s",
                    "particles": args.particles, "sim_seconds": targets[-1] - opt.t0,
                    "variant": args.variant, "integrator": "DOP853" if opt.integratorType == 0 else "RK45",
-                   "path": "persistent (st_run)"},
+                   "path": f"st_run, {args.engine} engine"},
         "plain_value": plain,
-        "roofline": {"bound": "mfma", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved_tf / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "k_engine<fast|exact, DOP853, persistent>",
-                     "algorithmic_flops": flops, "work": res["work"]},
+        "roofline": dict({"bound": "mfma", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS,
+                          "unit": "TFLOP/s", "frac": achieved_tf / FP64_PEAK_TFLOPS, "traffic": None,
+                          "kernel": kernel, "compute": "FP64 vector ALU (no MFMA); peak = FP64 vector rate",
+                          "work": w}, **roof_extra),
         "cpu_baseline": cpu,
         "failures": res["failures"],
     }

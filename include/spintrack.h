@@ -64,12 +64,16 @@ typedef struct st_options {
 #define ST_VARIANT_EXACT 0  /* the reference's operation order, no FMA contraction */
 #define ST_VARIANT_FAST  1  /* FMA-contracted, per-segment field polynomial; agrees
                                with EXACT to the integrator tolerance */
+#define ST_ENGINE_PIPELINE 0  /* kinematics kernel + spin kernel over staged segment records */
+#define ST_ENGINE_FUSED    1  /* one kernel interleaving events and integrator trials */
 typedef struct st_ext {
     int32_t pulse_axis;     /* axis of pulse(t) = a*cos(w*t): 0,1,2; reference = 2 (z),
                                src/integrator.cpp:40-42 */
     int32_t variant;        /* ST_VARIANT_* */
-    int32_t block_size;     /* threads per workgroup for the plain kernel; 0 = default */
-    int32_t reserved;
+    int32_t block_size;     /* threads per workgroup of the fused plain kernel; 0 = 256 */
+    int32_t engine;         /* ST_ENGINE_* */
+    int32_t seg_buffer_mb;  /* HBM budget of the pipeline's segment-record buffer; 0 = 16384 */
+    int32_t reserved[3];
 } st_ext;
 
 /* include/particle.h:52-59 */
@@ -160,10 +164,11 @@ ST_API int st_set_stop_times(st_handle* h, const double* stop_at);
 ST_API int st_get_state(st_handle* h, st_particle* out);
 ST_API int st_set_state(st_handle* h, const st_particle* in);
 
-/* ODE work counters summed since st_init: accepted trial steps, rejected
- * trial steps, integrated segments, and the number of event-loop trips.
- * (The reference keeps no ODE step counter, src/integrator.cpp:101.) */
-ST_API int st_work_counters(st_handle* h, uint64_t out[4]);
+/* Work counters summed since st_init (6 entries): accepted integrator trial
+ * steps, rejected trial steps, integrated segments, spin-loop trips, events
+ * (collision/move/sanity passes) and kinematic records written by the
+ * pipeline.  (The reference keeps no ODE step counter, src/integrator.cpp:101.) */
+ST_API int st_work_counters(st_handle* h, uint64_t out[6]);
 
 /* Per-particle ODE work since st_init (n = numParticles entries each; any
  * pointer may be NULL): accepted and rejected trial steps, and integrated
@@ -182,6 +187,25 @@ ST_API int st_particle_counters(st_handle* h, uint64_t* accepted, uint64_t* reje
 #define ST_TRACE_DOUBLES 20
 ST_API int st_set_trace(st_handle* h, int64_t particle, uint32_t capacity);
 ST_API int st_get_trace(st_handle* h, double* out, uint32_t capacity, uint32_t* n_records);
+
+/* Device memory for snapshot blocks and other buffers a caller hands to
+ * st_run / st_snapshot_block_device, without a second HIP runtime in the
+ * process; st_memcpy_d2h copies on the handle's stream and waits. */
+ST_API int st_device_alloc(size_t bytes, void** ptr);
+ST_API int st_device_free(void* ptr);
+ST_API int st_memcpy_d2h(st_handle* h, void* dst, const void* src, size_t bytes);
+
+/* Event timing on the handle's stream: st_timer_stop records the closing
+ * event, waits for it and returns the elapsed milliseconds. */
+ST_API int st_timer_start(st_handle* h);
+ST_API int st_timer_stop(st_handle* h, float* ms);
+
+/* Per-kernel timing of the pipeline engine: when enabled, every later
+ * advance/run records HIP events around each kinematics and spin kernel
+ * launch; st_kernel_times returns the summed milliseconds and the launch
+ * counts ([0] kinematics, [1] spin).  Enabling resets the sums. */
+ST_API int st_kernel_timing(st_handle* h, int enable);
+ST_API int st_kernel_times(st_handle* h, double ms[2], uint64_t launches[2]);
 
 /* Waits for all work queued on the handle's stream. */
 ST_API int st_synchronize(st_handle* h);

@@ -1,0 +1,472 @@
+// st_pipeline.h -- the two-kernel lane pipeline.
+//
+// The particle's motion never depends on its spin (the fields do not act
+// back on the trajectory), so the reference's event loop
+// (src/particle.cpp:754-803) separates into
+//   * k_kin:  the branchy kinematics -- collision times, moves, wall/gas
+//             reflections, RNG, sanity checks (src/particle.cpp:121-444) --
+//             which writes one 64-byte record per trajectory segment to HBM;
+//   * k_spin: the uniform FP64 work -- one DOP853/RK45 trial step per loop
+//             trip (src/integrator.cpp:90-337) -- which reads the records.
+// Every lane of a k_kin wave handles one event per trip, and a k_spin lane
+// only pays a record load and a segment set-up between trial steps, so
+// neither kernel runs the other's code on idle lanes.  Runs are cut into
+// chunks of at most C records per particle (C from the staging-buffer
+// budget); particles that finish drop out of the next chunk's active list.
+//
+// A spin-integrator failure (DOP853 nstep > nmax, RK45 h < hmin or nstep >
+// nmax) stops the reference's event loop at that segment, so k_fixup rewinds
+// the particle's kinematics to the chunk start and replays it exactly up to
+// the failing segment before applying the stop (rare path).
+#pragma once
+
+#include "st_state.h"
+
+namespace st {
+
+// record flags
+constexpr uint64_t R_INTEGRATE = 1u;     // integrate the spin over this segment
+constexpr uint64_t R_IEND = 2u;          // the open interval closes after this record
+constexpr uint64_t R_STOP = 4u;          // the particle stops after this record
+constexpr uint64_t R_FIRST = 8u;         // first segment of an interval: tempH = h
+constexpr uint64_t R_EMPTY = 16u;        // interval closes without any event
+// bits 8-15 coll_type, 16-23 wall_hit, 32-63 failure code applied at R_STOP
+
+struct SegRec {
+    double t;              // segment end time (t after the move)
+    double px, py, pz;     // pos_old: position at the segment start
+    double vx, vy, vz;     // v_old: velocity during the segment (pre-move)
+    uint64_t flags;
+};
+static_assert(sizeof(SegRec) == 64, "segment record is one 64-byte line");
+
+struct PipeArgs {
+    RunArgs R;
+    char* recs;              // N * C records
+    uint32_t C;              // record capacity per particle per chunk
+    const uint32_t* list;    // active particles this chunk
+    uint32_t n_list;
+    uint32_t* next_list;     // particles still active after this chunk
+    uint32_t* next_count;
+    uint32_t* seg_count;     // records written per particle this chunk
+    char* backup;            // kinematic state at chunk start (for k_fixup)
+    uint32_t* fail_list;     // (pid, record index, status) triples
+    uint32_t* fail_count;
+    uint32_t fail_cap;
+    int32_t integ;           // integratorType
+};
+
+// ---------------------------------------------------------------------------
+// kinematic lane state
+// ---------------------------------------------------------------------------
+struct KinLane {
+    uint32_t pid;
+    Kin K;
+    Rng rng;
+    uint64_t n_steps;
+    double tf;
+    int32_t k;           // open interval index
+    bool open, done;
+    uint32_t n_events;
+};
+
+// kinematic fields (the ones k_kin reads and writes); backed up per chunk
+constexpr int KIN_FIELDS[] = {F_T, F_TOLD, F_NEXTGAS, F_TF, F_PX, F_PY, F_PZ, F_POX, F_POY, F_POZ,
+                              F_VX, F_VY, F_VZ, F_VOX, F_VOY, F_VOZ, F_RX, F_RY, F_RZ, F_RW, F_SPARE,
+                              F_FLAGS, F_NBOUNCE, F_NCOLL, F_NSTEPS, F_KSTATE, F_NEVENTS};
+constexpr int N_KIN_FIELDS = sizeof(KIN_FIELDS) / sizeof(int);
+
+__device__ __forceinline__ void kin_load(const DevState& D, uint32_t i, KinLane& L) {
+    const uint32_t o = i * 8u;
+    L.pid = i;
+    L.K.t = D.ld(F_T, o);
+    L.K.t_old = D.ld(F_TOLD, o);
+    L.K.next_gas = D.ld(F_NEXTGAS, o);
+    L.K.dt = 0.0;
+    L.tf = D.ld(F_TF, o);
+    L.K.pos = d3{D.ld(F_PX, o), D.ld(F_PY, o), D.ld(F_PZ, o)};
+    L.K.pos_old = d3{D.ld(F_POX, o), D.ld(F_POY, o), D.ld(F_POZ, o)};
+    L.K.v = d3{D.ld(F_VX, o), D.ld(F_VY, o), D.ld(F_VZ, o)};
+    L.K.v_old = d3{D.ld(F_VOX, o), D.ld(F_VOY, o), D.ld(F_VOZ, o)};
+    L.rng.x = D.ld<uint64_t>(F_RX, o);
+    L.rng.y = D.ld<uint64_t>(F_RY, o);
+    L.rng.z = D.ld<uint64_t>(F_RZ, o);
+    L.rng.w = D.ld<uint64_t>(F_RW, o);
+    L.rng.spare = D.ld(F_SPARE, o);
+    const uint64_t fl = D.ld<uint64_t>(F_FLAGS, o);
+    L.K.failure = (int32_t)(uint32_t)fl;
+    L.K.stop = (fl >> 32) & 1;
+    L.rng.has = (fl >> 33) & 1;
+    L.K.coll_type = (char)(uint8_t)(fl >> 40);
+    L.K.wall_hit = (char)(uint8_t)(fl >> 48);
+    L.K.n_bounce = D.ld<uint64_t>(F_NBOUNCE, o);
+    L.K.n_coll = D.ld<uint64_t>(F_NCOLL, o);
+    L.n_steps = D.ld<uint64_t>(F_NSTEPS, o);
+    const uint64_t ks = D.ld<uint64_t>(F_KSTATE, o);
+    L.k = (int32_t)(uint32_t)ks;
+    L.open = (ks >> 32) & 1;
+    L.done = (ks >> 33) & 1;
+    L.n_events = (uint32_t)D.ld<uint64_t>(F_NEVENTS, o);
+    L.K.finished = false;
+}
+
+__device__ __forceinline__ void kin_store(const DevState& D, const KinLane& L) {
+    const uint32_t o = L.pid * 8u;
+    D.st(F_T, o, L.K.t);
+    D.st(F_TOLD, o, L.K.t_old);
+    D.st(F_NEXTGAS, o, L.K.next_gas);
+    D.st(F_TF, o, L.tf);
+    D.st(F_PX, o, L.K.pos.x); D.st(F_PY, o, L.K.pos.y); D.st(F_PZ, o, L.K.pos.z);
+    D.st(F_POX, o, L.K.pos_old.x); D.st(F_POY, o, L.K.pos_old.y); D.st(F_POZ, o, L.K.pos_old.z);
+    D.st(F_VX, o, L.K.v.x); D.st(F_VY, o, L.K.v.y); D.st(F_VZ, o, L.K.v.z);
+    D.st(F_VOX, o, L.K.v_old.x); D.st(F_VOY, o, L.K.v_old.y); D.st(F_VOZ, o, L.K.v_old.z);
+    D.st<uint64_t>(F_RX, o, L.rng.x);
+    D.st<uint64_t>(F_RY, o, L.rng.y);
+    D.st<uint64_t>(F_RZ, o, L.rng.z);
+    D.st<uint64_t>(F_RW, o, L.rng.w);
+    D.st(F_SPARE, o, L.rng.spare);
+    D.st<uint64_t>(F_FLAGS, o, pack_flags(L.K.failure, L.K.stop, L.rng.has, L.K.coll_type, L.K.wall_hit));
+    D.st<uint64_t>(F_NBOUNCE, o, L.K.n_bounce);
+    D.st<uint64_t>(F_NCOLL, o, L.K.n_coll);
+    D.st<uint64_t>(F_NSTEPS, o, L.n_steps);
+    D.st<uint64_t>(F_KSTATE, o, (uint64_t)(uint32_t)L.k | ((uint64_t)L.open << 32) | ((uint64_t)L.done << 33));
+    D.st<uint64_t>(F_NEVENTS, o, L.n_events);
+}
+
+__device__ __forceinline__ void kin_block(const RunArgs& R, const KinLane& L) {
+    if (R.blocks)
+        write_block_kin(R.blocks + (uint64_t)L.k * R.block_bytes, R.n, L.pid, L.K.t, L.K.pos, L.K.v, L.K.failure,
+                        L.K.n_coll, L.K.n_bounce, L.n_steps);
+}
+
+__device__ __forceinline__ void put_rec(char* recs, uint32_t C, uint32_t pid, uint32_t j, double t, const d3& p,
+                                        const d3& v, uint64_t flags) {
+    double* r = reinterpret_cast<double*>(recs + ((uint64_t)pid * C + j) * sizeof(SegRec));
+    r[0] = t; r[1] = p.x; r[2] = p.y; r[3] = p.z; r[4] = v.x; r[5] = v.y; r[6] = v.z;
+    reinterpret_cast<uint64_t*>(r)[7] = flags;
+}
+
+// One step of the kinematic state machine: opens an interval if needed,
+// runs one event of the reference's loop body, and returns the record that
+// describes it (or an R_EMPTY interval close).  Shared by k_kin and k_fixup.
+__device__ __forceinline__ uint64_t kin_step(const SimParams& P, const RunArgs& R, int32_t integ, KinLane& L,
+                                             double& rt, d3& rp, d3& rv) {
+    bool first = false;
+    if (!L.open) {
+        // src/particle.cpp:748-752: tf = nextTOut; finished = false; tempH = h
+        const double T = target_of(R, L.k);
+        const double prev = (L.k == 0) ? R.t_prev : target_of(R, L.k - 1);
+        bool moves = true;
+        double tgt = T;
+        if (R.stop_at) {
+            const double e = R.stop_at[L.pid];
+            if (!(prev < e)) moves = false;
+            else if (e < T) tgt = e;
+        }
+        if (!moves || L.K.stop) {
+            // the reference's loop body never runs this interval
+            if (moves) L.tf = tgt;
+            rt = L.K.t;
+            rp = L.K.pos;
+            rv = L.K.v;
+            kin_block(R, L);
+            L.k++;
+            if (L.k >= R.n_targets) L.done = true;
+            return R_EMPTY | R_IEND;
+        }
+        L.tf = tgt;
+        L.K.finished = false;
+        L.n_events = 0;
+        L.open = true;
+        first = true;
+    }
+    // src/particle.cpp:755-758
+    next_collision(P, L.tf, L.K, L.rng);
+    move_and_collide(P, L.K, L.rng);
+    sanity(P, L.K);
+    if (++L.n_events > R.max_events && L.K.failure == 0) {
+        L.K.stop = true;
+        L.K.failure = FAIL_WATCHDOG;
+    }
+    uint64_t fl = (first ? R_FIRST : 0) | ((uint64_t)(uint8_t)L.K.coll_type << 8) |
+                  ((uint64_t)(uint8_t)L.K.wall_hit << 16);
+    if (L.K.failure == 0 && integ != INT_NONE) fl |= R_INTEGRATE;
+    rt = L.K.t;
+    rp = L.K.pos_old;
+    rv = L.K.v_old;
+    if (L.K.stop) {
+        fl |= R_STOP | ((uint64_t)(uint32_t)L.K.failure << 32);
+        const double nan = __builtin_nan("");
+        L.K.t = nan;
+        L.K.pos = d3{nan, nan, nan};
+        L.K.v = d3{nan, nan, nan};
+    }
+    L.n_steps += 1;
+    if (L.K.finished || L.K.stop) {
+        fl |= R_IEND;
+        kin_block(R, L);
+        L.k++;
+        L.open = false;
+        if (L.k >= R.n_targets) L.done = true;
+    }
+    return fl;
+}
+
+// Resets both cursors at the start of an advance / run call.
+__global__ __launch_bounds__(256) void k_begin_run(const DevState D, uint32_t n, uint32_t* list) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t o = i * 8u;
+    D.st<uint64_t>(F_KSTATE, o, 0);
+    D.st<uint64_t>(F_SSTATE, o, 0);
+    D.st<uint64_t>(F_NEVENTS, o, 0);
+    // kinematics and spin are in step between calls: the first segment of
+    // this call starts at the particle's current time
+    D.st(F_TSPIN, o, D.ld(F_T, o));
+    list[i] = i;
+}
+
+__global__ __launch_bounds__(256) void k_kin(const SimParams P, const DevState D, const PipeArgs A) {
+    const uint32_t li = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long cnt[2] = {0, 0};   // events, trips
+    if (li < A.n_list) {
+        KinLane L;
+        kin_load(D, A.list[li], L);
+        // back up the kinematic state for a possible spin-failure replay
+        const uint32_t o = L.pid * 8u;
+        const DevState B{A.backup, D.stride};
+#pragma unroll
+        for (int f = 0; f < N_KIN_FIELDS; ++f) B.st<uint64_t>(KIN_FIELDS[f], o, D.ld<uint64_t>(KIN_FIELDS[f], o));
+        uint32_t j = 0;
+        while (j < A.C && !L.done) {
+            double rt;
+            d3 rp, rv;
+            const uint64_t fl = kin_step(P, A.R, A.integ, L, rt, rp, rv);
+            put_rec(A.recs, A.C, L.pid, j, rt, rp, rv, fl);
+            ++j;
+            cnt[0] += (fl & R_EMPTY) ? 0 : 1;
+        }
+        cnt[1] = j;
+        A.seg_count[L.pid] = j;
+        kin_store(D, L);
+        if (!L.done) A.next_list[atomicAdd(A.next_count, 1u)] = L.pid;
+    }
+    flush_counters(A.R.work + 4, cnt);
+}
+
+// ---------------------------------------------------------------------------
+// spin lane
+// ---------------------------------------------------------------------------
+struct SpinLane {
+    uint32_t pid, j, nrec;
+    int32_t k;           // interval index of the next snapshot block
+    bool stopped;
+    double h, tempH, t_spin;
+    d3 S;
+    uint32_t acc, rej, seg;
+};
+
+__device__ __forceinline__ void spin_load(const DevState& D, const PipeArgs& A, uint32_t pid, SpinLane& L) {
+    const uint32_t o = pid * 8u;
+    L.pid = pid;
+    L.j = 0;
+    L.nrec = A.seg_count[pid];
+    const uint64_t ss = D.ld<uint64_t>(F_SSTATE, o);
+    L.k = (int32_t)(uint32_t)ss;
+    L.stopped = (ss >> 33) & 1;
+    L.h = D.ld(F_H, o);
+    L.tempH = D.ld(F_TEMPH, o);
+    L.t_spin = D.ld(F_TSPIN, o);
+    L.S = d3{D.ld(F_SX, o), D.ld(F_SY, o), D.ld(F_SZ, o)};
+    L.acc = L.rej = L.seg = 0;
+}
+
+__device__ __forceinline__ void spin_store(const DevState& D, const SpinLane& L) {
+    const uint32_t o = L.pid * 8u;
+    D.st<uint64_t>(F_SSTATE, o, (uint64_t)(uint32_t)L.k | ((uint64_t)L.stopped << 33));
+    D.st(F_H, o, L.h);
+    D.st(F_TEMPH, o, L.tempH);
+    D.st(F_TSPIN, o, L.t_spin);
+    D.st(F_SX, o, L.S.x); D.st(F_SY, o, L.S.y); D.st(F_SZ, o, L.S.z);
+    D.st<uint64_t>(F_ACC, o, D.ld<uint64_t>(F_ACC, o) + L.acc);
+    D.st<uint64_t>(F_REJ, o, D.ld<uint64_t>(F_REJ, o) + L.rej);
+    D.st<uint64_t>(F_SEG, o, D.ld<uint64_t>(F_SEG, o) + L.seg);
+}
+
+__device__ __forceinline__ SegRec get_rec(const char* recs, uint32_t C, uint32_t pid, uint32_t j) {
+    const double* r = reinterpret_cast<const double*>(recs + ((uint64_t)pid * C + j) * sizeof(SegRec));
+    SegRec s;
+    s.t = r[0]; s.px = r[1]; s.py = r[2]; s.pz = r[3]; s.vx = r[4]; s.vy = r[5]; s.vz = r[6];
+    s.flags = reinterpret_cast<const uint64_t*>(r)[7];
+    return s;
+}
+
+__device__ __forceinline__ void spin_block(const RunArgs& R, const SpinLane& L) {
+    if (R.blocks) write_block_spin(R.blocks + (uint64_t)L.k * R.block_bytes, R.n, L.pid, L.S);
+}
+
+template <int VAR, int INTEG>
+__global__ __launch_bounds__(256) void k_spin(const SimParams P, const DevState D, const PipeArgs A) {
+    const RunArgs& R = A.R;
+    SpinLane L;
+    DopState dop;
+    RkState rk;
+    Seg seg;
+    Poly poly;
+    uint64_t cur_flags = 0;
+    double cur_t = 0.0;
+    unsigned long long cnt[4] = {0, 0, 0, 0};   // accepted, rejected, segments, trips
+    enum { S_DONE = 0, S_NEXT = 1, S_INT = 2 };
+    int phase = S_DONE;
+    {
+        const uint32_t q = acquire_index(R.queue);
+        if (q < A.n_list) {
+            spin_load(D, A, A.list[q], L);
+            phase = S_NEXT;
+        }
+    }
+    while (phase != S_DONE) {
+        ++cnt[3];
+        if (phase == S_NEXT) {
+            // consume records until one needs integration (or the chunk ends)
+            while (phase == S_NEXT) {
+                if (L.j >= L.nrec) {
+                    cnt[0] += L.acc; cnt[1] += L.rej; cnt[2] += L.seg;
+                    spin_store(D, L);
+                    phase = S_DONE;
+                    const uint32_t q = acquire_index(R.queue);
+                    if (q < A.n_list) {
+                        spin_load(D, A, A.list[q], L);
+                        phase = S_NEXT;
+                        continue;
+                    }
+                    break;
+                }
+                const SegRec rec = get_rec(A.recs, A.C, L.pid, L.j);
+                cur_flags = rec.flags;
+                cur_t = rec.t;
+                if (rec.flags & R_FIRST) L.tempH = L.h;
+                if ((rec.flags & R_INTEGRATE) && !L.stopped) {
+                    seg.t0 = L.t_spin;
+                    seg.p0 = d3{rec.px, rec.py, rec.pz};
+                    seg.v0 = d3{rec.vx, rec.vy, rec.vz};
+                    if (VAR == VAR_FAST) poly = make_poly(P, seg);
+                    if (INTEG == INT_DOP853) {
+                        if (VAR == VAR_EXACT) dop_begin_exact(P, dop, L.S, seg, rec.t);
+                        else dop_begin_fast(P, dop, L.S, seg, poly, rec.t);
+                    } else {
+                        rk_begin(rk, seg, rec.t, L.tempH);
+                    }
+                    L.seg++;
+                    if (R.trace && L.pid == R.trace_pid)
+                        ST_TRACE(R, 1, seg.t0, rec.t, seg.p0, seg.v0, L.S, L.tempH, L.acc, L.rej,
+                                     (char)(rec.flags >> 8), (char)(rec.flags >> 16), 0, 0, 0);
+                    phase = S_INT;
+                } else {
+                    // no integration: sanity failure, trajectory-only run, or an empty close
+                    if (rec.flags & R_STOP) {
+                        const double nan = __builtin_nan("");
+                        L.S = d3{nan, nan, nan};
+                    }
+                    if (rec.flags & R_IEND) {
+                        spin_block(R, L);
+                        L.k++;
+                    }
+                    if (!(rec.flags & R_EMPTY)) L.t_spin = rec.t;
+                    L.j++;
+                }
+            }
+        }
+        if (phase == S_INT) {
+            int st;
+            if (INTEG == INT_DOP853) {
+                if (VAR == VAR_EXACT) st = dop_trial_exact(P, dop, L.S, L.tempH, seg, L.acc, L.rej);
+                else st = dop_trial_fast(P, dop, L.S, L.tempH, seg, poly, L.acc, L.rej);
+            } else {
+                st = rk_trial<VAR>(P, rk, L.S, L.tempH, seg, poly, L.acc, L.rej);
+            }
+            if (st != 0) {
+                if (R.trace && L.pid == R.trace_pid)
+                    ST_TRACE(R, 2, L.t_spin, cur_t, d3{0, 0, 0}, d3{0, 0, 0}, L.S, L.tempH, L.acc, L.rej,
+                             (char)(cur_flags >> 8), (char)(cur_flags >> 16), st, 0, 0);
+                // src/particle.cpp:593-612
+                if (P.keep) L.h = L.tempH;
+                struct {
+                    uint64_t flags;
+                } rec{cur_flags};
+                L.t_spin = cur_t;
+                if (st < 0) {
+                    // the reference stops the particle here; the kinematics ran
+                    // ahead, so k_fixup replays it up to this record
+                    const uint32_t slot = atomicAdd(A.fail_count, 1u);
+                    if (slot < A.fail_cap) {
+                        A.fail_list[3 * slot + 0] = L.pid;
+                        A.fail_list[3 * slot + 1] = L.j;
+                        A.fail_list[3 * slot + 2] = (uint32_t)st;
+                    }
+                    L.stopped = true;
+                }
+                if (L.stopped || (rec.flags & R_STOP)) {
+                    const double nan = __builtin_nan("");
+                    L.S = d3{nan, nan, nan};
+                }
+                if ((rec.flags & R_IEND) || st < 0) {
+                    // (a failing particle's later blocks are written by k_fixup)
+                    spin_block(R, L);
+                    L.k++;
+                }
+                L.j++;
+                phase = S_NEXT;
+            }
+        }
+    }
+    flush_counters(R.work, cnt);
+}
+
+// Rare path: a spin-integration failure at record j_fail of this chunk.
+// Rewind the kinematics to the chunk start, replay exactly j_fail+1 records
+// (the failing segment's event included), then apply the reference's stop:
+// failureState = status, t/pos/v/S NaN, and every later snapshot block shows
+// the stopped particle.
+__global__ __launch_bounds__(64) void k_fixup(const SimParams P, const DevState D, const PipeArgs A,
+                                              uint32_t n_fail) {
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n_fail) return;
+    const uint32_t pid = A.fail_list[3 * f + 0];
+    const uint32_t jf = A.fail_list[3 * f + 1];
+    const int32_t status = (int32_t)A.fail_list[3 * f + 2];
+    const uint32_t o = pid * 8u;
+    const DevState B{A.backup, D.stride};
+    for (int q = 0; q < N_KIN_FIELDS; ++q) D.st<uint64_t>(KIN_FIELDS[q], o, B.ld<uint64_t>(KIN_FIELDS[q], o));
+    KinLane L;
+    kin_load(D, pid, L);
+    RunArgs R = A.R;
+    R.blocks = nullptr;   // blocks are rewritten below
+    int32_t k_at = L.k;
+    for (uint32_t j = 0; j <= jf; ++j) {
+        k_at = L.k;
+        double rt;
+        d3 rp, rv;
+        (void)kin_step(P, R, A.integ, L, rt, rp, rv);
+    }
+    // the failing segment belongs to interval k_at; the stop happens there
+    const double nan = __builtin_nan("");
+    L.K.stop = true;
+    L.K.failure = status;
+    L.K.t = nan;
+    L.K.pos = d3{nan, nan, nan};
+    L.K.v = d3{nan, nan, nan};
+    if (A.R.blocks) {
+        const d3 Sn{nan, nan, nan};
+        for (int32_t k = k_at; k < A.R.n_targets; ++k)
+            write_record(A.R.blocks + (uint64_t)k * A.R.block_bytes, A.R.n, pid, L.K.t, L.K.pos, L.K.v, Sn,
+                         L.K.failure, L.K.n_coll, L.K.n_bounce, L.n_steps);
+    }
+    L.k = A.R.n_targets;
+    L.open = false;
+    L.done = true;
+    kin_store(D, L);
+    D.st<uint64_t>(F_SSTATE, o, (uint64_t)(uint32_t)A.R.n_targets | (1ull << 33));
+}
+
+}  // namespace st

@@ -52,9 +52,13 @@ class Options(C.Structure):
 assert C.sizeof(Options) == 440, C.sizeof(Options)
 
 
+ENGINE_PIPELINE = 0
+ENGINE_FUSED = 1
+
+
 class Ext(C.Structure):
     _fields_ = [("pulse_axis", C.c_int32), ("variant", C.c_int32), ("block_size", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("engine", C.c_int32), ("seg_buffer_mb", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 class Rng(C.Structure):
@@ -99,7 +103,35 @@ SIGNATURES = {
     "st_synchronize": (C.c_int, [_vp]),
     "st_set_trace": (C.c_int, [_vp, C.c_int64, _u32]),
     "st_get_trace": (C.c_int, [_vp, _vp, _u32, C.POINTER(_u32)]),
+    "st_device_alloc": (C.c_int, [_sz, C.POINTER(_vp)]),
+    "st_device_free": (C.c_int, [_vp]),
+    "st_memcpy_d2h": (C.c_int, [_vp, _vp, _vp, _sz]),
+    "st_timer_start": (C.c_int, [_vp]),
+    "st_timer_stop": (C.c_int, [_vp, C.POINTER(C.c_float)]),
+    "st_kernel_timing": (C.c_int, [_vp, C.c_int]),
+    "st_kernel_times": (C.c_int, [_vp, _vp, _vp]),
 }
+
+
+class DeviceBuffer:
+    """Device memory owned through the library (its own HIP runtime)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(load().st_device_alloc(self.nbytes, C.byref(p)))
+        self.ptr = int(p.value)
+
+    def free(self) -> None:
+        if getattr(self, "ptr", 0):
+            load().st_device_free(C.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 TRACE_DOUBLES = 20
 TRACE_FIELDS = ["tag", "t_old", "t", "px", "py", "pz", "vx", "vy", "vz", "sx", "sy", "sz", "h",

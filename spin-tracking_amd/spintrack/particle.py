@@ -55,11 +55,11 @@ class Particle:
     """GPU-resident ensemble of ``opt.numParticles`` particles."""
 
     def __init__(self, opt: Options, pulse_axis: int = 2, variant: int = L.VARIANT_EXACT,
-                 block_size: int = 0):
+                 block_size: int = 0, engine: int = L.ENGINE_PIPELINE, seg_buffer_mb: int = 0):
         self.lib = L.load()
         self.opt = Options.from_buffer_copy(bytes(opt))
         self.n = int(opt.numParticles)
-        self.ext = Ext(int(pulse_axis), int(variant), int(block_size), 0)
+        self.ext = Ext(int(pulse_axis), int(variant), int(block_size), int(engine), int(seg_buffer_mb))
         h = C.c_void_p()
         check(self.lib.st_create(C.byref(self.opt), C.byref(self.ext), C.byref(h)), None)
         self.h = h
@@ -137,9 +137,10 @@ class Particle:
         check(self.lib.st_set_state(self.h, _ptr(s)), self.h)
 
     def work_counters(self) -> dict:
-        w = np.zeros(4, np.uint64)
+        w = np.zeros(6, np.uint64)
         check(self.lib.st_work_counters(self.h, _ptr(w)), self.h)
-        return dict(accepted=int(w[0]), rejected=int(w[1]), segments=int(w[2]), trips=int(w[3]))
+        return dict(accepted=int(w[0]), rejected=int(w[1]), segments=int(w[2]), trips=int(w[3]),
+                    events=int(w[4]), kin_records=int(w[5]))
 
     def particle_counters(self) -> dict:
         a, r, s = (np.zeros(self.n, np.uint64) for _ in range(3))
@@ -158,6 +159,30 @@ class Particle:
         n = C.c_uint32()
         check(self.lib.st_get_trace(self.h, _ptr(out), cap, C.byref(n)), self.h)
         return out[: n.value]
+
+    def copy_to_host(self, dev_ptr: int, nbytes: int) -> np.ndarray:
+        out = np.empty(int(nbytes), np.uint8)
+        check(self.lib.st_memcpy_d2h(self.h, _ptr(out), C.c_void_p(int(dev_ptr)), int(nbytes)), self.h)
+        return out
+
+    def kernel_timing(self, enable: bool = True) -> None:
+        check(self.lib.st_kernel_timing(self.h, 1 if enable else 0), self.h)
+
+    def kernel_times(self) -> dict:
+        ms = np.zeros(2)
+        n = np.zeros(2, np.uint64)
+        check(self.lib.st_kernel_times(self.h, _ptr(ms), _ptr(n)), self.h)
+        return dict(kin_ms=float(ms[0]), spin_ms=float(ms[1]), kin_launches=int(n[0]),
+                    spin_launches=int(n[1]))
+
+    def timer_start(self) -> None:
+        check(self.lib.st_timer_start(self.h), self.h)
+
+    def timer_stop(self) -> float:
+        """Milliseconds since timer_start, measured with events on this stream."""
+        ms = C.c_float()
+        check(self.lib.st_timer_stop(self.h, C.byref(ms)), self.h)
+        return float(ms.value)
 
     def set_stream(self, stream_ptr: int) -> None:
         check(self.lib.st_set_stream(self.h, stream_ptr or None), self.h)

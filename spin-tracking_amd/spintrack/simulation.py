@@ -37,17 +37,15 @@ def main_analysis(opt, output_name: str, pulse_axis: int = 2, variant: int = L.V
 
 def main_analysis_persistent(opt, output_name: str, pulse_axis: int = 2,
                              variant: int = L.VARIANT_FAST, tc_override=None) -> None:
-    import torch  # device memory for the snapshot blocks
-
     opt = derive(type(opt).from_buffer_copy(bytes(opt)), tc_override)
     targets = output_times(opt)
     with BinaryLogger(opt, output_name) as lg, Particle(opt, pulse_axis, variant) as p:
         p.initParticles()
         bb = p.block_bytes()
-        blocks = torch.empty((len(targets) + 1) * bb, dtype=torch.uint8, device="cuda")
-        p.snapshot_block_device(blocks.data_ptr())
-        p.run(targets, blocks.data_ptr() + bb)
-        p.synchronize()
-        host = blocks.cpu().numpy()
+        blocks = L.DeviceBuffer((len(targets) + 1) * bb)
+        p.snapshot_block_device(blocks.ptr)
+        p.run(targets, blocks.ptr + bb)
+        host = p.copy_to_host(blocks.ptr, blocks.nbytes)
+        blocks.free()
         for k in range(len(targets) + 1):
             lg.writeBlock(host[k * bb:(k + 1) * bb])

@@ -1,15 +1,21 @@
 """GPU parity: the HIP path (through the C ABI) against the CPU oracle.
 
 Bar (the tolerances are the ones stated for this path):
-  * integer state bit-exact: bounce, gas-collision and segment counts,
-    failure codes, the RNG words, and the number of accepted / rejected
-    integrator trial steps per particle (identical event and step sequences);
-  * spin and position within 1e-9 absolute (exact variant: agreement is
-    ulp-level; libm transcendentals -- cos, sin, acos, log, pow -- are ocml on
-    the GPU and glibc on the CPU and may differ in the last bit);
-  * the fast variant (FMA contraction, per-segment field polynomial) within
-    1e-9 of the oracle, event sequences identical.
+  * integer state bit-exact: bounce, gas-collision and event counts, failure
+    codes, the RNG words, integrated-segment counts (identical event
+    sequences);
+  * spin and position within 1e-9 absolute for runs without gas collisions
+    or diffuse walls (the exact variant agrees to ~1e-14 there);
+  * with gas collisions and diffuse walls, libm last-bit differences (ocml on
+    the GPU, glibc on the CPU: log in the exponential/normal draws, sin/cos/
+    acos in diffuse re-emission) are amplified by grazing wall hits under
+    gravity; there the spread must stay within that of the reference
+    algorithm against itself with a 1-ulp perturbation (ulp_control), and
+    every traced segment, replayed through the oracle's integrator from the
+    GPU's own inputs, must agree to 1e-13 (chaos-free local parity).
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -20,10 +26,12 @@ from conftest import to_oracle_opt
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-9
+PIPE, FUSED = st._lib.ENGINE_PIPELINE, st._lib.ENGINE_FUSED
 
 
-def run_gpu(opt, targets, variant=st.VARIANT_EXACT, persistent=False, pulse_axis=2, stop_at=None):
-    p = st.Particle(opt, pulse_axis=pulse_axis, variant=variant)
+def run_gpu(opt, targets, variant=st.VARIANT_EXACT, persistent=False, pulse_axis=2, stop_at=None,
+            engine=PIPE, seg_buffer_mb=0):
+    p = st.Particle(opt, pulse_axis=pulse_axis, variant=variant, engine=engine, seg_buffer_mb=seg_buffer_mb)
     p.initParticles()
     if stop_at is not None:
         p.set_stop_times(stop_at)
@@ -48,87 +56,43 @@ def run_oracle(O, opt, targets, pulse_axis=2, stop_at=None):
     return E
 
 
-def compare(O, g, gstate, gcnt, E, tol=TOL, check_steps=True):
+def check_integer_state(O, g, gstate, gcnt, E):
     r = E.snapshot()
-    a = E.arr
     for k in ("n_bounce", "n_coll", "n_steps", "failure"):
         bad = np.nonzero(g[k] != r[k])[0]
         assert bad.size == 0, f"{k} differs for {bad.size} particles, first {bad[:5]}"
     for k in ("x", "y", "z", "w"):
-        assert np.array_equal(gstate[f"rng_{k}"], a["state"][k]), f"rng word {k} differs"
-    assert np.array_equal(gstate["hasSpare"].astype(bool), a["state"]["hasSpare"])
+        assert np.array_equal(gstate[f"rng_{k}"], E.arr["state"][k]), f"rng word {k} differs"
+    assert np.array_equal(gstate["hasSpare"].astype(bool), E.arr["state"]["hasSpare"])
+    assert np.array_equal(gcnt["segments"], E.n_ode.reshape(-1, 3)[:, 2]), "segment counts differ"
+    return r
+
+
+def compare(O, g, gstate, gcnt, E, tol=TOL):
+    r = check_integer_state(O, g, gstate, gcnt, E)
     n_ode = E.n_ode.reshape(-1, 3)
-    # segments are event-driven: identical event sequences => identical counts
-    assert np.array_equal(gcnt["segments"], n_ode[:, 2]), "segment counts differ"
     out = {}
-    # Accepted/rejected trial counts follow the step-size controller, which
-    # sees ulp-level differences once a libm value (log in the normal draws,
-    # pow in the controller, sin/cos in diffuse re-emission) differs in its
-    # last bit; an accept/reject flip then shifts a count and the carried
-    # step size.  Reported, and bounded loosely.
+    # accepted/rejected trial counts follow the step-size controller, which
+    # sees last-bit libm differences (pow in the controller, log in the
+    # initial normal draws); reported, and bounded loosely
     da = np.abs(gcnt["accepted"].astype(np.int64) - n_ode[:, 0].astype(np.int64))
-    dr = np.abs(gcnt["rejected"].astype(np.int64) - n_ode[:, 1].astype(np.int64))
-    out["steps_identical"] = float(np.mean((da == 0) & (dr == 0)))
-    out["max_dacc_rel"] = float((da / np.maximum(1, n_ode[:, 0])).max())
-    if check_steps:
-        assert out["max_dacc_rel"] < 0.05, f"accepted counts: max relative diff {out['max_dacc_rel']}"
+    out["steps_identical"] = float(np.mean(da == 0))
+    assert (da / np.maximum(1, n_ode[:, 0])).max() < 0.05
     for k in ("S", "pos", "v"):
-        d = np.abs(st.xyz(g[k]) - O.xyz(r[k]))
+        a, b = st.xyz(g[k]), O.xyz(r[k])
+        assert np.array_equal(np.isfinite(a), np.isfinite(b)), f"{k} NaN pattern"
+        d = np.abs(a - b)
         fin = np.isfinite(d)
-        assert np.array_equal(np.isfinite(st.xyz(g[k])), np.isfinite(O.xyz(r[k]))), f"{k} NaN pattern"
         mx = d[fin].max() if fin.any() else 0.0
-        if mx >= tol:
-            dd = np.where(fin, d, 0).max(axis=1)
-            worst = np.argsort(dd)[::-1][:5]
-            for i in worst:
-                print(f"  {k} particle {i}: |d|={dd[i]:.3e} gpu={st.xyz(g[k])[i]} ref={O.xyz(r[k])[i]} "
-                      f"acc gpu/ref={gcnt['accepted'][i]}/{n_ode[i, 0]} rej={gcnt['rejected'][i]}/{n_ode[i, 1]} "
-                      f"seg={gcnt['segments'][i]} coll={g['n_coll'][i]} bounce={g['n_bounce'][i]}")
-            print(f"  particles above tol: {(dd >= tol).sum()} of {dd.size}")
         assert mx < tol, f"max |d{k}| = {mx:.3e}"
         out[k] = mx
-    dt = np.abs(g["t"] - r["t"])
-    assert np.nanmax(dt) < 1e-12
-    out["bitexact_S"] = float(np.mean(np.all(st.xyz(g["S"]) == O.xyz(r["S"]), axis=1)))
+    assert np.nanmax(np.abs(g["t"] - r["t"])) < 1e-12 if np.isfinite(g["t"]).any() else True
     return out
-
-
-def test_neutron_cell_exact_advance(oracle):
-    """1000 neutrons, 1 s, 20 output intervals, exact variant, st_advance_to."""
-    opt = configs.neutron_cell()
-    targets = st.output_times(opt)
-    g, gs, gc = run_gpu(opt, targets)
-    E = run_oracle(oracle, opt, targets)
-    res = compare(oracle, g, gs, gc, E)
-    print("neutron_cell exact:", res)
-
-
-def test_neutron_cell_persistent_equals_advance():
-    """st_run (persistent, one launch) == st_advance_to per interval, bit for bit."""
-    opt = configs.neutron_cell(n=777, tf=0.5)
-    targets = st.output_times(opt)
-    for variant in (st.VARIANT_EXACT, st.VARIANT_FAST):
-        g1, s1, c1 = run_gpu(opt, targets, variant=variant)
-        g2, s2, c2 = run_gpu(opt, targets, variant=variant, persistent=True)
-        assert s1.tobytes() == s2.tobytes(), f"variant {variant}: state differs"
-        for k in c1:
-            assert np.array_equal(c1[k], c2[k])
-
-
-def test_neutron_cell_fast(oracle):
-    opt = configs.neutron_cell(n=500)
-    targets = st.output_times(opt)
-    g, gs, gc = run_gpu(opt, targets, variant=st.VARIANT_FAST)
-    E = run_oracle(oracle, opt, targets)
-    res = compare(oracle, g, gs, gc, E, check_steps=False)
-    print("neutron_cell fast:", res)
 
 
 def ulp_control(O, opt, targets):
     """The reference algorithm against itself with each particle's first gas
-    time moved by one ulp: the spread libm last-bit differences produce once
-    grazing wall hits (sqrt near zero in the hit time under gravity) amplify
-    them.  Returns per-particle max |dS| and max |dpos|."""
+    time moved by one ulp; returns per-particle max |dS| and max |dpos|."""
     runs = []
     for perturb in (False, True):
         E = O.Ensemble(to_oracle_opt(opt), opt.numParticles)
@@ -144,15 +108,7 @@ def ulp_control(O, opt, targets):
 
 
 def compare_chaotic(O, opt, targets, g, gs, gc, E):
-    """Parity for runs with gas collisions and diffuse walls: integer state
-    and event sequences bit-exact; |dS|, |dpos| within the spread of the
-    1-ulp control (at most twice its fraction above 1e-9, plus 0.5%)."""
-    r = E.snapshot()
-    for k in ("n_bounce", "n_coll", "n_steps", "failure"):
-        assert np.array_equal(g[k], r[k]), f"{k} differs"
-    for k in ("x", "y", "z", "w"):
-        assert np.array_equal(gs[f"rng_{k}"], E.arr["state"][k]), f"rng word {k} differs"
-    assert np.array_equal(gc["segments"], E.n_ode.reshape(-1, 3)[:, 2])
+    r = check_integer_state(O, g, gs, gc, E)
     dS = np.abs(st.xyz(g["S"]) - O.xyz(r["S"])).max(1)
     dp = np.abs(st.xyz(g["pos"]) - O.xyz(r["pos"])).max(1)
     cS, cp = ulp_control(O, opt, targets)
@@ -163,6 +119,45 @@ def compare_chaotic(O, opt, targets, g, gs, gc, E):
     assert out["frac_dpos_gt_1e9"] <= 2 * out["ctrl_frac_dpos"] + 0.005, out
     assert out["median_dS"] < 1e-12, out
     return out
+
+
+# --------------------------------------------------------------------------
+@pytest.mark.parametrize("engine", [PIPE, FUSED])
+def test_neutron_cell_exact_advance(oracle, engine):
+    """1000 neutrons, 1 s, 20 output intervals, exact variant, st_advance_to."""
+    opt = configs.neutron_cell()
+    targets = st.output_times(opt)
+    g, gs, gc = run_gpu(opt, targets, engine=engine)
+    E = run_oracle(oracle, opt, targets)
+    print(f"neutron_cell exact engine={engine}:", compare(oracle, g, gs, gc, E))
+
+
+def test_neutron_cell_fast(oracle):
+    opt = configs.neutron_cell(n=500)
+    targets = st.output_times(opt)
+    g, gs, gc = run_gpu(opt, targets, variant=st.VARIANT_FAST, persistent=True)
+    E = run_oracle(oracle, opt, targets)
+    print("neutron_cell fast:", compare(oracle, g, gs, gc, E))
+
+
+@pytest.mark.parametrize("variant", [st.VARIANT_EXACT, st.VARIANT_FAST])
+def test_engines_and_paths_agree_bitwise(variant):
+    """Pipeline (with a tiny staging buffer, so runs cross many chunk
+    boundaries) and fused engines, persistent and per-interval paths: the
+    same per-particle state, bit for bit."""
+    opt = configs.he3_thermal(n=777, tf=0.5, ioutInt=0.125)
+    targets = st.output_times(opt)
+    ref = None
+    for engine, persistent, mb in ((FUSED, False, 0), (FUSED, True, 0), (PIPE, False, 1), (PIPE, True, 1),
+                                   (PIPE, True, 0)):
+        g, s, c = run_gpu(opt, targets, variant=variant, persistent=persistent, engine=engine,
+                          seg_buffer_mb=mb)
+        if ref is None:
+            ref = (s, c)
+            continue
+        assert s.tobytes() == ref[0].tobytes(), (engine, persistent, mb)
+        for k in c:
+            assert np.array_equal(c[k], ref[1][k]), (engine, persistent, mb, k)
 
 
 def test_he3_thermal_exact(oracle):
@@ -185,10 +180,9 @@ def test_he3_thermal_fast_persistent(oracle):
 
 @pytest.mark.parametrize("variant", [st.VARIANT_EXACT, st.VARIANT_FAST])
 def test_he3_segment_replay(oracle, variant):
-    """Chaos-free local parity: every segment the GPU integrated for a traced
-    particle, replayed through the oracle's integrateDOP from the GPU's own
-    segment inputs (t_old, t, pos_old, v_old, S, step size), ends at the
-    GPU's spin within 1e-13 (exact) / 1e-11 (fast)."""
+    """Every segment the GPU integrated for a traced particle, replayed
+    through the oracle's integrateDOP from the GPU's own segment inputs,
+    ends at the GPU's spin within 1e-13 (exact) / 1e-11 (fast)."""
     opt = configs.he3_thermal(n=2000, tf=1.0, ioutInt=0.25)
     o = to_oracle_opt(opt)
     for pid in (3, 1271):
@@ -208,3 +202,87 @@ def test_he3_segment_replay(oracle, variant):
             worst = max(worst, np.abs(y - e[9:12]).max())
         print(f"variant {variant} particle {pid}: {len(starts)} segments, worst |dS| {worst:.2e}")
         assert worst < (1e-13 if variant == st.VARIANT_EXACT else 1e-11)
+
+
+@pytest.mark.parametrize("engine", [PIPE, FUSED])
+def test_integrator_failure_nmax(oracle, engine):
+    """nmax = 4: segments needing more than 5 trial steps stop the particle
+    with failureState -1 (src/integrator.cpp:129-131, src/particle.cpp:596-612).
+    The pipeline must rewind its kinematics to the failing event."""
+    opt = configs.he3_thermal(n=300, tf=0.3, ioutInt=0.1)
+    opt.nmax = 4
+    targets = st.output_times(opt)
+    g, gs, gc = run_gpu(opt, targets, engine=engine, seg_buffer_mb=1)
+    E = run_oracle(oracle, opt, targets)
+    r = E.snapshot()
+    assert (r["failure"] == -1).sum() > 20 and (r["failure"] == 0).sum() > 0
+    check_integer_state(oracle, g, gs, gc, E)
+    fin = r["failure"] == 0
+    assert np.all(np.isnan(st.xyz(g["S"])[~fin]))
+    assert np.all(np.isnan(g["t"][~fin]))
+
+
+def test_rk45_exact(oracle):
+    """integratorType 1 with the stale-k1 quirk, neutrons and He-3."""
+    for opt in (configs.neutron_cell(n=256, tf=0.02, ioutInt=0.01),
+                configs.he3_thermal(n=256, tf=0.02, ioutInt=0.01, integrator=1)):
+        opt.integratorType = 1
+        targets = st.output_times(opt)
+        g, gs, gc = run_gpu(opt, targets)
+        E = run_oracle(oracle, opt, targets)
+        res = compare(oracle, g, gs, gc, E) if opt.gas_coll == 0 else compare_chaotic(
+            oracle, opt, targets, g, gs, gc, E)
+        print("rk45:", res)
+
+
+def test_pulse_along_y(oracle):
+    """Dressing pulse a*cos(w*t) along y (pulse hook on another axis)."""
+    # output times off the 3 kHz pulse's zero phase, where the dressed spin
+    # is furthest from yi
+    opt = configs.dressing(n=300, species="n", tf=0.05, ioutInt=0.01241)
+    opt.gas_coll = 0
+    opt.diffuse = 0.0
+    targets = st.output_times(opt)
+    for variant in (st.VARIANT_EXACT, st.VARIANT_FAST):
+        g, gs, gc = run_gpu(opt, targets, variant=variant, pulse_axis=1)
+        E = run_oracle(oracle, opt, targets, pulse_axis=1)
+        print(f"dressing variant {variant}:", compare(oracle, g, gs, gc, E, tol=TOL))
+        # the pulse dominates: the spin must have moved far from yi
+        assert np.abs(st.xyz(g["S"])[:, 0] - 1).max() > 1e-3
+
+
+def test_stop_times_and_t0(oracle):
+    """Per-particle end times (uneven durations) and a non-zero t0."""
+    opt = configs.neutron_cell(n=400, tf=1.0, ioutInt=0.25)
+    opt.t0 = 0.1
+    rng = np.random.default_rng(2)
+    stop_at = rng.uniform(0.1, 1.0, 400)
+    targets = st.output_times(opt)
+    g, gs, gc = run_gpu(opt, targets, stop_at=stop_at, persistent=True)
+    E = run_oracle(oracle, opt, targets, stop_at=stop_at)
+    print("stop times:", compare(oracle, g, gs, gc, E))
+    assert np.allclose(g["t"], np.minimum(stop_at, targets[-1]), atol=1e-12)
+
+
+def test_snapshot_blocks_match_snapshots():
+    """st_run's on-GPU snapshot blocks == st_snapshot after each st_advance_to."""
+    opt = configs.he3_thermal(n=301, tf=0.4, ioutInt=0.1)
+    targets = st.output_times(opt)
+    with st.Particle(opt, variant=st.VARIANT_FAST) as p:
+        p.initParticles()
+        bb = p.block_bytes()
+        blocks = st._lib.DeviceBuffer((len(targets) + 1) * bb)
+        p.snapshot_block_device(blocks.ptr)
+        p.run(targets, blocks.ptr + bb)
+        host = p.copy_to_host(blocks.ptr, blocks.nbytes)
+        blocks.free()
+    dt = st.snapshot_block_dtype(301)
+    with st.Particle(opt, variant=st.VARIANT_FAST) as q:
+        q.initParticles()
+        for k in range(len(targets) + 1):
+            if k:
+                q.runSimulation(targets[k - 1])
+            s = q.snapshot()
+            b = np.frombuffer(host[k * bb:(k + 1) * bb].tobytes(), dtype=dt)[0]
+            for f in ("t", "pos", "v", "S", "failure", "n_coll", "n_bounce", "n_steps"):
+                assert b[f].tobytes() == s[f].tobytes(), (k, f)
