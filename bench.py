@@ -38,6 +38,11 @@ FLOPS_EVENT = 75.0
 FLOPS_RK45_TRIAL = 650.0
 FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 vector, dense
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per segment record moved by k_spin (reads + writes), from the
+# FETCH_SIZE / WRITE_SIZE counters of the pipeline on this workload, scaled by
+# a 1 GiB device copy (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000); the
+# algorithmic figure is 64 B per record plus the per-particle spin state.
+SPIN_HBM_BYTES_PER_RECORD = 71.7
 
 
 def build_workload(n, integrator):
@@ -73,6 +78,15 @@ def run_gpu(opt, args, targets, persistent, variant, engine):
             w.synchronize()
     p = st.Particle(opt, **kw)
     p.initParticles()
+    if args.copy_calibration_mb and persistent:
+        # one device-to-device copy of known size: the reference dispatch
+        # that scales the FETCH_SIZE / WRITE_SIZE counters to bytes
+        nb = args.copy_calibration_mb << 20
+        a, b = st._lib.DeviceBuffer(nb), st._lib.DeviceBuffer(nb)
+        p.copy_device(b.ptr, a.ptr, nb)
+        p.synchronize()
+        a.free()
+        b.free()
     bb = p.block_bytes()
     blocks = st._lib.DeviceBuffer(len(targets) * bb)
     p.synchronize()
@@ -134,9 +148,12 @@ def main():
     ap.add_argument("--integrator", type=int, default=0, choices=[0, 1])
     ap.add_argument("--no-plain", action="store_true", help="skip the per-interval (plain) run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
-    ap.add_argument("--cpu-sample", type=int, default=8192, help="particles in the CPU sample")
+    ap.add_argument("--cpu-sample", type=int, default=16384, help="particles in the CPU sample")
     ap.add_argument("--engine", choices=["pipeline", "fused"], default="pipeline")
     ap.add_argument("--seg-buffer-mb", type=int, default=0, help="pipeline staging budget (0 = library default)")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="issue one device-to-device copy of this many MiB before the timed run "
+                         "(reference dispatch for hardware byte counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -167,15 +184,19 @@ def main():
         spin_flops = (FLOPS_RK45_TRIAL * (w["accepted"] + w["rejected"]) if opt.integratorType == 1
                       else fa * w["accepted"] + FLOPS_REJ * w["rejected"])
         per_launch_ms = kt["spin_ms"] / max(1, kt["spin_launches"])
+        traffic = SPIN_HBM_BYTES_PER_RECORD * w["kin_records"] / max(1, kt["spin_launches"])
         achieved_tf = spin_flops / (kt["spin_ms"] / 1e3) / 1e12
         kernel = f"k_spin<{args.variant}, {'RK45' if opt.integratorType == 1 else 'DOP853'}>"
         roof_extra = {"launches": kt["spin_launches"], "avg_launch_ms": per_launch_ms,
                       "kernel_ms": kt["spin_ms"], "kin_kernel_ms": kt["kin_ms"],
-                      "kin_launches": kt["kin_launches"], "whole_run_TFLOPs": flops / (res["kernel_ms"] / 1e3) / 1e12}
+                      "kin_launches": kt["kin_launches"], "whole_run_TFLOPs": flops / (res["kernel_ms"] / 1e3) / 1e12,
+                      "traffic_unit": "HBM bytes per k_spin launch (measured bytes per record x records per launch)",
+                      "traffic_GBs": traffic / (per_launch_ms / 1e3) / 1e9}
     else:
         achieved_tf = flops / (res["kernel_ms"] / 1e3) / 1e12
         kernel = f"k_engine<{args.variant}, persistent>"
         roof_extra = {"launches": 1, "avg_launch_ms": res["kernel_ms"]}
+        traffic = None
     print(f"[bench] persistent: {res['kernel_ms']:.1f} ms, {value:.4g} particle-s/s, kernels={kt}, "
           f"work={w}", file=sys.stderr, flush=True)
 
@@ -187,7 +208,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(opt, args.particles, args.cpu_sample, min(2, steps))
+        cpu = cpu_baseline(opt, args.particles, args.cpu_sample, min(4, steps))
         print(f"[bench] cpu: {cpu}", file=sys.stderr, flush=True)
 
     line = {
@@ -210,7 +231,7 @@ def main():
                    "path": f"st_run, {args.engine} engine"},
         "plain_value": plain,
         "roofline": dict({"bound": "mfma", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS,
-                          "unit": "TFLOP/s", "frac": achieved_tf / FP64_PEAK_TFLOPS, "traffic": None,
+                          "unit": "TFLOP/s", "frac": achieved_tf / FP64_PEAK_TFLOPS, "traffic": traffic,
                           "kernel": kernel, "compute": "FP64 vector ALU (no MFMA); peak = FP64 vector rate",
                           "work": w}, **roof_extra),
         "cpu_baseline": cpu,

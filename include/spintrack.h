@@ -194,6 +194,8 @@ ST_API int st_get_trace(st_handle* h, double* out, uint32_t capacity, uint32_t* 
 ST_API int st_device_alloc(size_t bytes, void** ptr);
 ST_API int st_device_free(void* ptr);
 ST_API int st_memcpy_d2h(st_handle* h, void* dst, const void* src, size_t bytes);
+/* Device-to-device copy queued on the handle's stream (asynchronous). */
+ST_API int st_memcpy_d2d(st_handle* h, void* dst, const void* src, size_t bytes);
 
 /* Event timing on the handle's stream: st_timer_stop records the closing
  * event, waits for it and returns the elapsed milliseconds. */

@@ -714,6 +714,12 @@ ST_API int st_memcpy_d2h(st_handle* h, void* dst, const void* src, size_t bytes)
     return ST_OK;
 }
 
+ST_API int st_memcpy_d2d(st_handle* h, void* dst, const void* src, size_t bytes) {
+    if (!h || !dst || !src) return fail(h, ST_E_ARG, "null argument");
+    HIPCHK(h, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, h->stream));
+    return ST_OK;
+}
+
 ST_API int st_timer_start(st_handle* h) {
     if (!h) return fail(nullptr, ST_E_ARG, "null handle");
     if (!h->ev[0]) {

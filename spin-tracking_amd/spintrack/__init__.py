@@ -14,3 +14,5 @@ from ._lib import (VARIANT_EXACT, VARIANT_FAST, Coords, Ext, Options, SpinTrackE
 from .options import (HELIUM3, NEUTRON, default_options, derive, n_intervals,  # noqa: F401
                       output_times, parse_params, read_params, set_fields)
 from .particle import COORDS, Particle, snapshot_block_dtype, xyz  # noqa: F401
+from .logger import BinaryLogger, read_output  # noqa: F401
+from . import configs, simulation  # noqa: F401

@@ -106,6 +106,7 @@ SIGNATURES = {
     "st_device_alloc": (C.c_int, [_sz, C.POINTER(_vp)]),
     "st_device_free": (C.c_int, [_vp]),
     "st_memcpy_d2h": (C.c_int, [_vp, _vp, _vp, _sz]),
+    "st_memcpy_d2d": (C.c_int, [_vp, _vp, _vp, _sz]),
     "st_timer_start": (C.c_int, [_vp]),
     "st_timer_stop": (C.c_int, [_vp, C.POINTER(C.c_float)]),
     "st_kernel_timing": (C.c_int, [_vp, C.c_int]),

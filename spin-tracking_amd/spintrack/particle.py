@@ -165,6 +165,10 @@ class Particle:
         check(self.lib.st_memcpy_d2h(self.h, _ptr(out), C.c_void_p(int(dev_ptr)), int(nbytes)), self.h)
         return out
 
+    def copy_device(self, dst_ptr: int, src_ptr: int, nbytes: int) -> None:
+        check(self.lib.st_memcpy_d2d(self.h, C.c_void_p(int(dst_ptr)), C.c_void_p(int(src_ptr)),
+                                     int(nbytes)), self.h)
+
     def kernel_timing(self, enable: bool = True) -> None:
         check(self.lib.st_kernel_timing(self.h, 1 if enable else 0), self.h)
 
