@@ -110,6 +110,70 @@ def run_gpu(opt, args, targets, persistent, variant, engine):
     return dict(kernel_ms=kernel_ms, wall_s=wall, work=w, failures=fails, ktimes=kt)
 
 
+def bench_sweep(args, variant):
+    """Field-configuration sweep: 64 configurations x 16,384 neutrons with
+    their own gradients, Ez polarity and wall model, and per-particle
+    durations uniform in [1, 20] s.  Batched = the whole sweep in one
+    handle and one st_run call (persistent kernels over all configurations);
+    plain = the drop-in path as the reference's loop drives it, one
+    configuration after the other, st_advance_to per output interval.  Both
+    are timed with HIP events on the library's stream; particle-seconds =
+    the sum of the per-particle durations."""
+    import numpy as np
+    import spintrack as st
+
+    cfgs, stop_at = st.configs.batched_sweep(n_cfg=args.sweep_configs, n_per=args.sweep_per)
+    targets = st.output_times(cfgs[0])
+    psec = float(np.sum(stop_at - cfgs[0].t0))
+    # warm-up: load both kernel families once
+    wc, ws = st.configs.batched_sweep(n_cfg=2, n_per=1024, tf=1.0, ioutInt=0.5, dur=(0.5, 1.0))
+    with st.Particle.batch(wc, variant=variant) as w:
+        w.initParticles()
+        w.set_stop_times(ws)
+        w.run(st.output_times(wc[0]))
+        w.synchronize()
+    with st.Particle(wc[0], variant=variant) as w:
+        w.initParticles()
+        w.runSimulation(0.5)
+        w.synchronize()
+
+    b = st.Particle.batch(cfgs, variant=variant)
+    b.initParticles()
+    b.set_stop_times(stop_at)
+    b.synchronize()
+    b.kernel_timing(True)
+    b.timer_start()
+    b.run(targets)
+    batched_ms = b.timer_stop()
+    kt = b.kernel_times()
+    wk = b.work_counters()
+    fails = int((b.snapshot()["failure"] != 0).sum())
+    b.close()
+    spin_flops = FLOPS_ACC_GRAV * wk["accepted"] + FLOPS_REJ * wk["rejected"]
+    spin_tf = spin_flops / (kt["spin_ms"] / 1e3) / 1e12
+
+    plain_ms = 0.0
+    off = np.cumsum([0] + [c.numParticles for c in cfgs])
+    for c, cfg in enumerate(cfgs):
+        with st.Particle(cfg, variant=variant) as p:
+            p.initParticles()
+            p.set_stop_times(stop_at[off[c]:off[c + 1]])
+            p.synchronize()
+            p.timer_start()
+            for t in targets:
+                p.runSimulation(t)
+            plain_ms += p.timer_stop()
+    batched = psec / (batched_ms / 1e3)
+    plain = psec / (plain_ms / 1e3)
+    return {"workload": f"{len(cfgs)} field configurations x {args.sweep_per} neutrons, durations U[1,20] s, "
+                        f"random gradients in [-5e-8,5e-8] T/m, Ez +-7.5e6 V/m, half diffuse 0.1",
+            "particle_seconds": psec, "batched_value": batched, "batched_ms": batched_ms,
+            "plain_value": plain, "plain_ms": plain_ms, "batched_over_plain": batched / plain,
+            "batched_spin_TFLOPs": spin_tf, "batched_spin_frac_fp64_peak": spin_tf / FP64_PEAK_TFLOPS,
+            "batched_whole_run_TFLOPs": (spin_flops + FLOPS_EVENT * wk["segments"]) / (batched_ms / 1e3) / 1e12,
+            "kernels": kt, "work": wk, "failures": fails}
+
+
 def cpu_baseline(opt, n_total, sample_particles, sample_intervals):
     """The CPU oracle (the reference CPU path restated in C, OpenMP over
     particles) on a deterministic subset: every k-th particle of the same
@@ -151,6 +215,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=16384, help="particles in the CPU sample")
     ap.add_argument("--engine", choices=["pipeline", "fused"], default="pipeline")
     ap.add_argument("--seg-buffer-mb", type=int, default=0, help="pipeline staging budget (0 = library default)")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the batched field-sweep leg")
+    ap.add_argument("--sweep-configs", type=int, default=64)
+    ap.add_argument("--sweep-per", type=int, default=16384, help="neutrons per sweep configuration")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="issue one device-to-device copy of this many MiB before the timed run "
                          "(reference dispatch for hardware byte counters)")
@@ -211,6 +278,11 @@ def main():
         cpu = cpu_baseline(opt, args.particles, args.cpu_sample, min(4, steps))
         print(f"[bench] cpu: {cpu}", file=sys.stderr, flush=True)
 
+    sweep = None
+    if not args.no_sweep:
+        sweep = bench_sweep(args, variant)
+        print(f"[bench] sweep: {sweep}", file=sys.stderr, flush=True)
+
     line = {
         "metric": "particle-seconds/s, 1 MI355X FP64 rtol=1e-12",
         "value": value,
@@ -236,6 +308,7 @@ def main():
                           "work": w}, **roof_extra),
         "cpu_baseline": cpu,
         "failures": res["failures"],
+        "batched_sweep": sweep,
     }
     print(json.dumps(line))
 

@@ -104,6 +104,24 @@ typedef struct st_handle st_handle;
  * mainAnalysis derives them, src/simulation.cpp:9-11).  ext may be NULL. */
 ST_API int st_create(const st_options* opt, const st_ext* ext, st_handle** out);
 
+/* Batched form (no counterpart in the reference, whose particle class holds
+ * one options block, include/particle.h:90-166): n_cfg configurations in one
+ * handle, particles of configuration c at indices offsets[c] ..
+ * offsets[c+1]-1 (cfgs[c].numParticles each, back to back).  Per
+ * configuration: B0, E, Gx/Gy/Gz, gamma, pulse a and w, tc, sqrtKT_m,
+ * diffuse, V, yi, dist and seed; every other option (geometry, time axis,
+ * integrator control, gravity, gas_coll, step-size switches) must equal
+ * configuration 0's.  Particle j of configuration c is seeded as particle j
+ * of a plain run of cfgs[c], so each configuration's particles reproduce that
+ * plain run bit for bit.  All calls that take a handle work on a batch; the
+ * whole batch advances together and snapshot blocks cover all particles in
+ * index order.  Pipeline engine only. */
+ST_API int st_create_batch(const st_options* cfgs, int32_t n_cfg, const st_ext* ext, st_handle** out);
+
+/* Configuration layout: n_cfg and the n_cfg+1 particle offsets (a plain
+ * handle reports one configuration).  Either pointer may be NULL. */
+ST_API int st_batch_layout(const st_handle* h, int32_t* n_cfg, uint32_t* offsets);
+
 /* == particle::~particle (include/particle.h:168-240) */
 ST_API void st_destroy(st_handle* h);
 

@@ -68,22 +68,35 @@ ST_HD __forceinline__ double max_d(double a, double b) { return (a > b) ? a : b;
 // Uniform (SGPR-resident) run parameters, derived once on the host from the
 // reference options block.
 // ---------------------------------------------------------------------------
-struct SimParams {
-    double Lx, Ly, Lz;
+// Physics of one field configuration / population: what the fields, the
+// species and the wall/gas models are.  A plain handle has one (in the
+// uniform kernel-argument block); a batched handle keeps one per
+// configuration in HBM and every lane reads its particle's.
+struct Phys {
     double B0x, B0y, B0z;
     double Ex, Ey, Ez;
     double G[9];          // rows Gx, Gy, Gz (grad, src/integrator.cpp:44-46)
     double gamma;
     double pa, pw;        // pulse amplitude and angular frequency
-    double rtol, atol, beta, safe, fac1, fac2, hmax, hmin, h0;
-    double expo1, facc1, facc2;   // integrateDOP set-up (src/integrator.cpp:105-108)
-    double maxPosStep, tc, sqrtKT_m, diffuse;
-    uint32_t nmax;
+    double tc, sqrtKT_m, diffuse, V;
+    double yix, yiy, yiz; // initial spin
     int32_t pulse_axis;
     int32_t pulse_on;     // 0: the pulse term is +0 and may be skipped bit-exactly
-    int32_t pow_beta_on;  // 0: beta == 0, pow(facold, beta) == 1 exactly
+    int32_t diffuse_on;
     int32_t dist;         // 'C' or 'M'
-    int32_t gravity, gas_coll, fixed, keep, diffuse_on;
+};
+
+// Run parameters shared by every particle of a launch (geometry, integrator
+// control) plus the physics of a plain handle.
+struct SimParams {
+    double Lx, Ly, Lz;
+    double rtol, atol, beta, safe, fac1, fac2, hmax, hmin, h0;
+    double expo1, facc1, facc2;   // integrateDOP set-up (src/integrator.cpp:105-108)
+    double maxPosStep;
+    uint32_t nmax;
+    int32_t pow_beta_on;  // 0: beta == 0, pow(facold, beta) == 1 exactly
+    int32_t gravity, gas_coll, fixed, keep;
+    Phys ph;
 };
 
 // ---------------------------------------------------------------------------
@@ -189,7 +202,7 @@ struct Kin {
 // src/particle.cpp:121-257 -- closed-form wall hit times (quadratic in y
 // under gravity), the else-if fix-up chain, tie rules, gas vs wall, and the
 // maxPosStep / interval-end cap.
-ST_HD inline void next_collision(const SimParams& P, double tf, Kin& K, Rng& rng) {
+ST_HD inline void next_collision(const SimParams& P, const Phys& F, double tf, Kin& K, Rng& rng) {
     const d3 v = K.v, pos = K.pos;
     const double t = K.t;
     double dx, dy, dz, dtx, dty, dtz;
@@ -265,7 +278,7 @@ ST_HD inline void next_collision(const SimParams& P, double tf, Kin& K, Rng& rng
         gas_next = true;
         tcol = tgas;
     } else if (tgas == tb) {
-        K.next_gas += expo_draw(rng, P.tc);
+        K.next_gas += expo_draw(rng, F.tc);
         tcol = tgas;
         gas_next = false;
     } else {
@@ -286,7 +299,7 @@ ST_HD inline void next_collision(const SimParams& P, double tf, Kin& K, Rng& rng
         if (gas_next) {
             K.coll_type = 'G';
             K.n_coll += 1;
-            K.next_gas += expo_draw(rng, P.tc);
+            K.next_gas += expo_draw(rng, F.tc);
         } else {
             K.n_bounce += 1;
             K.coll_type = 'W';
@@ -307,7 +320,7 @@ ST_HD __forceinline__ void flip_v(d3& v, char wh) {
 }
 
 // src/particle.cpp:263-388
-ST_HD inline void move_and_collide(const SimParams& P, Kin& K, Rng& rng) {
+ST_HD inline void move_and_collide(const SimParams& P, const Phys& F, Kin& K, Rng& rng) {
     K.v_old = K.v;
     K.t_old = K.t;
     K.t += K.dt;
@@ -329,7 +342,7 @@ ST_HD inline void move_and_collide(const SimParams& P, Kin& K, Rng& rng) {
         if (wh == 'y' || wh == 'a' || wh == 'b' || wh == 'd') K.pos.y = psgn(K.pos.y) * P.Ly * 0.5;
         if (wh == 'z' || wh == 'a' || wh == 'c' || wh == 'd') K.pos.z = psgn(K.pos.z) * P.Lz * 0.5;
         bool diffuse = false;
-        if (P.diffuse_on) diffuse = uni(rng) < P.diffuse;
+        if (F.diffuse_on) diffuse = uni(rng) < F.diffuse;
         if (diffuse) {
             // Lambertian re-emission at the post-flight speed; the reference
             // draws phi and theta before it looks at the wall id.
@@ -357,13 +370,13 @@ ST_HD inline void move_and_collide(const SimParams& P, Kin& K, Rng& rng) {
     } else if (ct == 'G') {
         // 'M': three Maxwell-Boltzmann components; 'C': isotropic direction at
         // the current speed.  Both draw three normals in the same order.
-        if (P.dist == 'M' || P.dist == 'C') {
+        if (F.dist == 'M' || F.dist == 'C') {
             double n0, n1, n2;
             normal3(rng, n0, n1, n2);
-            if (P.dist == 'M') {
-                K.v.x = n0 * P.sqrtKT_m;
-                K.v.y = n1 * P.sqrtKT_m;
-                K.v.z = n2 * P.sqrtKT_m;
+            if (F.dist == 'M') {
+                K.v.x = n0 * F.sqrtKT_m;
+                K.v.y = n1 * F.sqrtKT_m;
+                K.v.z = n2 * F.sqrtKT_m;
             } else {
                 const double vn = len3(d3{n0, n1, n2});
                 K.v.x = (Vel * n0) / vn;
@@ -402,7 +415,7 @@ struct Seg {
 
 // Exact: interpolate -> grad -> pulse + B0 + (1/c^2) v x E + G.r -> gamma*B ->
 // y x (gamma B), in the reference's order.
-ST_HD __forceinline__ d3 rhs_exact(const SimParams& P, double t, const d3& y, const Seg& S) {
+ST_HD __forceinline__ d3 rhs_exact(const SimParams& P, const Phys& F, double t, const d3& y, const Seg& S) {
     const double d = t - S.t0;
     d3 p, v;
     if (P.gravity) {
@@ -418,23 +431,23 @@ ST_HD __forceinline__ d3 rhs_exact(const SimParams& P, double t, const d3& y, co
         p.z = S.p0.z + S.v0.z * d;
         v = S.v0;
     }
-    const double gx = p.x * P.G[0] + p.y * P.G[1] + p.z * P.G[2];
-    const double gy = p.x * P.G[3] + p.y * P.G[4] + p.z * P.G[5];
-    const double gz = p.x * P.G[6] + p.y * P.G[7] + p.z * P.G[8];
+    const double gx = p.x * F.G[0] + p.y * F.G[1] + p.z * F.G[2];
+    const double gy = p.x * F.G[3] + p.y * F.G[4] + p.z * F.G[5];
+    const double gz = p.x * F.G[6] + p.y * F.G[7] + p.z * F.G[8];
     double plx = 0.0, ply = 0.0, plz = 0.0;
-    if (P.pulse_on) {
-        const double pv = P.pa * cos(P.pw * t);
-        if (P.pulse_axis == 0) plx = pv;
-        else if (P.pulse_axis == 1) ply = pv;
+    if (F.pulse_on) {
+        const double pv = F.pa * cos(F.pw * t);
+        if (F.pulse_axis == 0) plx = pv;
+        else if (F.pulse_axis == 1) ply = pv;
         else plz = pv;
     }
-    const double cx = v.y * P.Ez - v.z * P.Ey;
-    const double cy = v.z * P.Ex - v.x * P.Ez;
-    const double cz = v.x * P.Ey - v.y * P.Ex;
-    const double Bx = ((plx + P.B0x) + cx * IC2) + gx;
-    const double By = ((ply + P.B0y) + cy * IC2) + gy;
-    const double Bz = ((plz + P.B0z) + cz * IC2) + gz;
-    const d3 w{Bx * P.gamma, By * P.gamma, Bz * P.gamma};
+    const double cx = v.y * F.Ez - v.z * F.Ey;
+    const double cy = v.z * F.Ex - v.x * F.Ez;
+    const double cz = v.x * F.Ey - v.y * F.Ex;
+    const double Bx = ((plx + F.B0x) + cx * IC2) + gx;
+    const double By = ((ply + F.B0y) + cy * IC2) + gy;
+    const double Bz = ((plz + F.B0z) + cz * IC2) + gz;
+    const d3 w{Bx * F.gamma, By * F.gamma, Bz * F.gamma};
     return cross3(y, w);
 }
 
@@ -445,41 +458,41 @@ struct Poly {
     d3 w0, w1, w2;
 };
 
-ST_HD inline Poly make_poly(const SimParams& P, const Seg& S) {
+ST_HD inline Poly make_poly(const SimParams& P, const Phys& F, const Seg& S) {
 #pragma clang fp contract(fast)
     const double ay = P.gravity ? G_CONST : 0.0;
     // B0 + G.p0 + (1/c^2) v0 x E
-    const double g0x = S.p0.x * P.G[0] + S.p0.y * P.G[1] + S.p0.z * P.G[2];
-    const double g0y = S.p0.x * P.G[3] + S.p0.y * P.G[4] + S.p0.z * P.G[5];
-    const double g0z = S.p0.x * P.G[6] + S.p0.y * P.G[7] + S.p0.z * P.G[8];
-    const double c0x = S.v0.y * P.Ez - S.v0.z * P.Ey;
-    const double c0y = S.v0.z * P.Ex - S.v0.x * P.Ez;
-    const double c0z = S.v0.x * P.Ey - S.v0.y * P.Ex;
+    const double g0x = S.p0.x * F.G[0] + S.p0.y * F.G[1] + S.p0.z * F.G[2];
+    const double g0y = S.p0.x * F.G[3] + S.p0.y * F.G[4] + S.p0.z * F.G[5];
+    const double g0z = S.p0.x * F.G[6] + S.p0.y * F.G[7] + S.p0.z * F.G[8];
+    const double c0x = S.v0.y * F.Ez - S.v0.z * F.Ey;
+    const double c0y = S.v0.z * F.Ex - S.v0.x * F.Ez;
+    const double c0z = S.v0.x * F.Ey - S.v0.y * F.Ex;
     // G.v0 + (1/c^2) a x E, a = (0, ay, 0)
-    const double g1x = S.v0.x * P.G[0] + S.v0.y * P.G[1] + S.v0.z * P.G[2];
-    const double g1y = S.v0.x * P.G[3] + S.v0.y * P.G[4] + S.v0.z * P.G[5];
-    const double g1z = S.v0.x * P.G[6] + S.v0.y * P.G[7] + S.v0.z * P.G[8];
-    const double c1x = ay * P.Ez, c1z = -ay * P.Ex;
+    const double g1x = S.v0.x * F.G[0] + S.v0.y * F.G[1] + S.v0.z * F.G[2];
+    const double g1y = S.v0.x * F.G[3] + S.v0.y * F.G[4] + S.v0.z * F.G[5];
+    const double g1z = S.v0.x * F.G[6] + S.v0.y * F.G[7] + S.v0.z * F.G[8];
+    const double c1x = ay * F.Ez, c1z = -ay * F.Ex;
     Poly q;
-    const double gm = P.gamma;
-    q.w0 = d3{gm * (P.B0x + g0x + c0x * IC2), gm * (P.B0y + g0y + c0y * IC2),
-              gm * (P.B0z + g0z + c0z * IC2)};
+    const double gm = F.gamma;
+    q.w0 = d3{gm * (F.B0x + g0x + c0x * IC2), gm * (F.B0y + g0y + c0y * IC2),
+              gm * (F.B0z + g0z + c0z * IC2)};
     q.w1 = d3{gm * (g1x + c1x * IC2), gm * g1y, gm * (g1z + c1z * IC2)};
     // 0.5 * G.a
-    q.w2 = d3{gm * (0.5 * ay * P.G[1]), gm * (0.5 * ay * P.G[4]), gm * (0.5 * ay * P.G[7])};
+    q.w2 = d3{gm * (0.5 * ay * F.G[1]), gm * (0.5 * ay * F.G[4]), gm * (0.5 * ay * F.G[7])};
     return q;
 }
 
-ST_HD __forceinline__ d3 rhs_fast(const SimParams& P, double t, const d3& y, double t0,
+ST_HD __forceinline__ d3 rhs_fast(const SimParams& P, const Phys& F, double t, const d3& y, double t0,
                                        const Poly& q) {
 #pragma clang fp contract(fast)
     const double tau = t - t0;
     d3 w{q.w0.x + tau * (q.w1.x + tau * q.w2.x), q.w0.y + tau * (q.w1.y + tau * q.w2.y),
          q.w0.z + tau * (q.w1.z + tau * q.w2.z)};
-    if (P.pulse_on) {
-        const double pv = P.gamma * (P.pa * cos(P.pw * t));
-        if (P.pulse_axis == 0) w.x += pv;
-        else if (P.pulse_axis == 1) w.y += pv;
+    if (F.pulse_on) {
+        const double pv = F.gamma * (F.pa * cos(F.pw * t));
+        if (F.pulse_axis == 0) w.x += pv;
+        else if (F.pulse_axis == 1) w.y += pv;
         else w.z += pv;
     }
     return cross3(y, w);

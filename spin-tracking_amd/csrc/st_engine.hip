@@ -40,13 +40,31 @@ static_assert(sizeof(st_rng) == 48, "st_rng must match the reference's 48-byte r
 namespace st {
 
 // initParticlesCPU (src/particle.cpp:638-710), one lane per particle; local
-// slot j is global particle first + j*stride.
+// slot j is global particle first + j*stride.  A batched handle lays its
+// configurations out back to back (cfg_off[c] .. cfg_off[c+1]); slot j of
+// configuration c is seeded as particle j of a plain run with that
+// configuration's seed, so each configuration reproduces its own plain run.
+template <bool BATCH>
 __global__ __launch_bounds__(256) void k_init(const SimParams P, const DevState D, uint32_t n, uint32_t seed,
-                                              uint32_t first, uint32_t stride, double V, double t0, d3 yi,
-                                              double tf0, int32_t dist) {
+                                              uint32_t first, uint32_t stride, double t0, double tf0,
+                                              const Phys* phys, const uint32_t* cfg_off, const uint32_t* cfg_seed,
+                                              uint32_t n_cfg) {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
-    const uint32_t ipart = first + j * stride;
+    uint32_t cfg = 0, local = j;
+    if constexpr (BATCH) {
+        uint32_t lo = 0, hi = n_cfg;   // last c with cfg_off[c] <= j
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (cfg_off[mid] <= j) lo = mid;
+            else hi = mid;
+        }
+        cfg = lo;
+        local = j - cfg_off[cfg];
+        seed = cfg_seed[cfg];
+    }
+    const Phys& F = BATCH ? phys[cfg] : P.ph;
+    const uint32_t ipart = first + local * stride;
     Rng rng;
     seed_rng(rng, seed, ipart);
     d3 p;
@@ -55,18 +73,19 @@ __global__ __launch_bounds__(256) void k_init(const SimParams P, const DevState 
     p.z = uni(rng) * P.Lz - P.Lz / 2.0;
     double ng;
     if (P.gas_coll)
-        ng = expo_draw(rng, P.tc);
+        ng = expo_draw(rng, F.tc);
     else
         ng = DBL_MAX_D;
     d3 v{0.0, 0.0, 0.0};
-    if (dist == 'C' || dist == 'M') {
+    if (F.dist == 'C' || F.dist == 'M') {
         double n0, n1, n2;
         normal3(rng, n0, n1, n2);
-        if (dist == 'C') {
+        if (F.dist == 'C') {
+            const double V = F.V;
             const double vn = len3(d3{n0, n1, n2});
             v = d3{(V * n0) / vn, (V * n1) / vn, (V * n2) / vn};
         } else {
-            v = d3{n0 * P.sqrtKT_m, n1 * P.sqrtKT_m, n2 * P.sqrtKT_m};
+            v = d3{n0 * F.sqrtKT_m, n1 * F.sqrtKT_m, n2 * F.sqrtKT_m};
         }
     }
     const uint32_t o = j * 8u;
@@ -80,7 +99,7 @@ __global__ __launch_bounds__(256) void k_init(const SimParams P, const DevState 
     D.st<uint64_t>(F_RZ, o, rng.z);
     D.st<uint64_t>(F_RW, o, rng.w);
     D.st(F_SPARE, o, rng.spare);
-    D.st(F_SX, o, yi.x); D.st(F_SY, o, yi.y); D.st(F_SZ, o, yi.z);
+    D.st(F_SX, o, F.yix); D.st(F_SY, o, F.yiy); D.st(F_SZ, o, F.yiz);
     D.st(F_T, o, t0);
     D.st(F_TOLD, o, t0);
     D.st(F_TF, o, tf0);
@@ -99,6 +118,7 @@ __global__ __launch_bounds__(256) void k_init(const SimParams P, const DevState 
     D.st<uint64_t>(F_KSTATE, o, 0);
     D.st<uint64_t>(F_SSTATE, o, 0);
     D.st<uint64_t>(F_NEVENTS, o, 0);
+    D.st<uint64_t>(F_CFG, o, cfg);
 }
 
 // Current state -> one snapshot block (reference file layout).
@@ -161,6 +181,13 @@ struct st_handle {
     hipEvent_t kev[4] = {nullptr, nullptr, nullptr, nullptr};
     double kms[2] = {0.0, 0.0};     // k_kin, k_spin milliseconds
     uint64_t klaunch[2] = {0, 0};   // launches
+    // batched handles: configurations back to back in particle order
+    bool batch = false;
+    uint32_t n_cfg = 1;
+    std::vector<uint32_t> cfg_off;   // n_cfg + 1 offsets
+    Phys* d_phys = nullptr;
+    uint32_t* d_cfg_off = nullptr;
+    uint32_t* d_cfg_seed = nullptr;
     std::string err;
 };
 
@@ -184,23 +211,36 @@ static int fail(st_handle* h, int code, const char* fmt, ...) {
             return fail((h), ST_E_HIP, "%s failed: %s", #call, hipGetErrorString(e_)); \
     } while (0)
 
+static Phys derive_phys(const st_options& o, const st_ext& x) {
+    Phys F;
+    memset(&F, 0, sizeof F);
+    F.B0x = o.B0.x; F.B0y = o.B0.y; F.B0z = o.B0.z;
+    F.Ex = o.E.x; F.Ey = o.E.y; F.Ez = o.E.z;
+    F.G[0] = o.Gx.x; F.G[1] = o.Gx.y; F.G[2] = o.Gx.z;
+    F.G[3] = o.Gy.x; F.G[4] = o.Gy.y; F.G[5] = o.Gy.z;
+    F.G[6] = o.Gz.x; F.G[7] = o.Gz.y; F.G[8] = o.Gz.z;
+    F.gamma = o.gamma;
+    F.pa = o.a;
+    F.pw = o.w;
+    F.pulse_axis = x.pulse_axis;
+    const double b0ax = x.pulse_axis == 0 ? o.B0.x : (x.pulse_axis == 1 ? o.B0.y : o.B0.z);
+    // a*cos(w*t) with a == 0 is a signed zero; added to B0[axis] it changes
+    // nothing unless B0[axis] is itself -0.0 (or cos is not finite).
+    F.pulse_on = (o.a != 0.0 || signbit(b0ax) || !isfinite(o.w)) ? 1 : 0;
+    F.tc = o.tc;
+    F.sqrtKT_m = o.sqrtKT_m;
+    F.diffuse = o.diffuse;
+    F.diffuse_on = (o.diffuse > (double)1.17549435082228750797e-38) ? 1 : 0;
+    F.V = o.V;
+    F.yix = o.yi.x; F.yiy = o.yi.y; F.yiz = o.yi.z;
+    F.dist = (int32_t)(unsigned char)o.dist;
+    return F;
+}
+
 static SimParams derive_params(const st_options& o, const st_ext& x) {
     SimParams P;
     memset(&P, 0, sizeof P);
     P.Lx = o.L.x; P.Ly = o.L.y; P.Lz = o.L.z;
-    P.B0x = o.B0.x; P.B0y = o.B0.y; P.B0z = o.B0.z;
-    P.Ex = o.E.x; P.Ey = o.E.y; P.Ez = o.E.z;
-    P.G[0] = o.Gx.x; P.G[1] = o.Gx.y; P.G[2] = o.Gx.z;
-    P.G[3] = o.Gy.x; P.G[4] = o.Gy.y; P.G[5] = o.Gy.z;
-    P.G[6] = o.Gz.x; P.G[7] = o.Gz.y; P.G[8] = o.Gz.z;
-    P.gamma = o.gamma;
-    P.pa = o.a;
-    P.pw = o.w;
-    P.pulse_axis = x.pulse_axis;
-    const double b0ax = x.pulse_axis == 0 ? o.B0.x : (x.pulse_axis == 1 ? o.B0.y : o.B0.z);
-    // a*cos(w*t) with a == 0 is a signed zero; added to B0[axis] it changes
-    // nothing unless B0[axis] is itself -0.0 (or cos is not finite).
-    P.pulse_on = (o.a != 0.0 || signbit(b0ax) || !isfinite(o.w)) ? 1 : 0;
     P.rtol = o.rtol; P.atol = o.atol; P.beta = o.beta; P.safe = o.safe;
     P.fac1 = o.fac1; P.fac2 = o.fac2; P.hmax = o.hmax; P.hmin = o.hmin; P.h0 = o.h;
     P.expo1 = 1.0 / 8.0 - o.beta * 0.2;
@@ -208,16 +248,12 @@ static SimParams derive_params(const st_options& o, const st_ext& x) {
     P.facc2 = 1.0 / o.fac2;
     P.pow_beta_on = (o.beta != 0.0) ? 1 : 0;
     P.maxPosStep = o.maxPosStep;
-    P.tc = o.tc;
-    P.sqrtKT_m = o.sqrtKT_m;
-    P.diffuse = o.diffuse;
-    P.diffuse_on = (o.diffuse > (double)1.17549435082228750797e-38) ? 1 : 0;
     P.nmax = o.nmax;
-    P.dist = (int32_t)(unsigned char)o.dist;
     P.gravity = o.gravity ? 1 : 0;
     P.gas_coll = o.gas_coll ? 1 : 0;
     P.fixed = o.fixedStepSize ? 1 : 0;
     P.keep = o.keepStepSize ? 1 : 0;
+    P.ph = derive_phys(o, x);
     return P;
 }
 
@@ -249,20 +285,31 @@ static const void* fused_symbol(int variant, int integ) {
                       : (const void*)&k_engine<VAR_EXACT, INT_RK45, true>;
 }
 
-template <int VAR, int INTEG>
+template <int VAR, int INTEG, bool BATCH>
 static hipError_t launch_spin(st_handle* h, dim3 grid, const PipeArgs& A) {
-    hipLaunchKernelGGL((k_spin<VAR, INTEG>), grid, dim3(256), 0, h->stream, h->P, h->D, A);
+    hipLaunchKernelGGL((k_spin<VAR, INTEG, BATCH>), grid, dim3(256), 0, h->stream, h->P, h->D, A);
     return hipGetLastError();
 }
 using SpinFn = hipError_t (*)(st_handle*, dim3, const PipeArgs&);
-static SpinFn pick_spin(int variant, int integ) {
-    if (variant == ST_VARIANT_FAST) return integ == 1 ? launch_spin<VAR_FAST, INT_RK45> : launch_spin<VAR_FAST, INT_DOP853>;
-    return integ == 1 ? launch_spin<VAR_EXACT, INT_RK45> : launch_spin<VAR_EXACT, INT_DOP853>;
-}
-static const void* spin_symbol(int variant, int integ) {
+template <bool BATCH>
+static SpinFn pick_spin_b(int variant, int integ) {
     if (variant == ST_VARIANT_FAST)
-        return integ == 1 ? (const void*)&k_spin<VAR_FAST, INT_RK45> : (const void*)&k_spin<VAR_FAST, INT_DOP853>;
-    return integ == 1 ? (const void*)&k_spin<VAR_EXACT, INT_RK45> : (const void*)&k_spin<VAR_EXACT, INT_DOP853>;
+        return integ == 1 ? launch_spin<VAR_FAST, INT_RK45, BATCH> : launch_spin<VAR_FAST, INT_DOP853, BATCH>;
+    return integ == 1 ? launch_spin<VAR_EXACT, INT_RK45, BATCH> : launch_spin<VAR_EXACT, INT_DOP853, BATCH>;
+}
+static SpinFn pick_spin(int variant, int integ, bool batch) {
+    return batch ? pick_spin_b<true>(variant, integ) : pick_spin_b<false>(variant, integ);
+}
+template <bool BATCH>
+static const void* spin_symbol_b(int variant, int integ) {
+    if (variant == ST_VARIANT_FAST)
+        return integ == 1 ? (const void*)&k_spin<VAR_FAST, INT_RK45, BATCH>
+                          : (const void*)&k_spin<VAR_FAST, INT_DOP853, BATCH>;
+    return integ == 1 ? (const void*)&k_spin<VAR_EXACT, INT_RK45, BATCH>
+                      : (const void*)&k_spin<VAR_EXACT, INT_DOP853, BATCH>;
+}
+static const void* spin_symbol(int variant, int integ, bool batch) {
+    return batch ? spin_symbol_b<true>(variant, integ) : spin_symbol_b<false>(variant, integ);
 }
 
 static int persistent_grid(st_handle* h, const void* sym, int* cache) {
@@ -314,7 +361,8 @@ static int run_pipeline(st_handle* h, RunArgs R) {
     int r = ensure_pipeline(h);
     if (r) return r;
     const int integ = h->opt.integratorType;
-    if (integ != INT_NONE && (r = persistent_grid(h, spin_symbol(h->ext.variant, integ), &h->grid_spin))) return r;
+    if (integ != INT_NONE && (r = persistent_grid(h, spin_symbol(h->ext.variant, integ, h->batch), &h->grid_spin)))
+        return r;
     const int bs = 256;
     hipLaunchKernelGGL(k_begin_run, dim3((h->n + bs - 1) / bs), dim3(bs), 0, h->stream, h->D, h->n, h->d_list[0]);
     HIPCHK(h, hipGetLastError());
@@ -330,16 +378,20 @@ static int run_pipeline(st_handle* h, RunArgs R) {
     A.fail_count = h->d_small + 2;
     A.next_count = h->d_small + 1;
     A.integ = integ;
+    A.phys = h->d_phys;
     uint32_t n_list = h->n;
     int cur = 0;
-    const SpinFn spin = integ == INT_NONE ? nullptr : pick_spin(h->ext.variant, integ);
+    const SpinFn spin = integ == INT_NONE ? nullptr : pick_spin(h->ext.variant, integ, h->batch);
     while (n_list > 0) {
         A.list = h->d_list[cur];
         A.n_list = n_list;
         A.next_list = h->d_list[cur ^ 1];
         HIPCHK(h, hipMemsetAsync(h->d_small, 0, 4 * sizeof(uint32_t), h->stream));
         if (h->ktime_on) HIPCHK(h, hipEventRecord(h->kev[0], h->stream));
-        hipLaunchKernelGGL(k_kin, dim3((n_list + bs - 1) / bs), dim3(bs), 0, h->stream, h->P, h->D, A);
+        if (h->batch)
+            hipLaunchKernelGGL(k_kin<true>, dim3((n_list + bs - 1) / bs), dim3(bs), 0, h->stream, h->P, h->D, A);
+        else
+            hipLaunchKernelGGL(k_kin<false>, dim3((n_list + bs - 1) / bs), dim3(bs), 0, h->stream, h->P, h->D, A);
         HIPCHK(h, hipGetLastError());
         if (h->ktime_on) HIPCHK(h, hipEventRecord(h->kev[1], h->stream));
         if (spin) {
@@ -366,7 +418,12 @@ static int run_pipeline(st_handle* h, RunArgs R) {
         }
         const uint32_t n_fail = counts[2];
         if (n_fail > 0) {
-            hipLaunchKernelGGL(k_fixup, dim3((n_fail + 63) / 64), dim3(64), 0, h->stream, h->P, h->D, A, n_fail);
+            if (h->batch)
+                hipLaunchKernelGGL(k_fixup<true>, dim3((n_fail + 63) / 64), dim3(64), 0, h->stream, h->P, h->D, A,
+                                   n_fail);
+            else
+                hipLaunchKernelGGL(k_fixup<false>, dim3((n_fail + 63) / 64), dim3(64), 0, h->stream, h->P, h->D, A,
+                                   n_fail);
             HIPCHK(h, hipGetLastError());
         }
         n_list = counts[1];
@@ -399,16 +456,7 @@ ST_API const char* st_version(void) { return "spintrack gfx950 exact,fast dop853
 
 ST_API const char* st_last_error(const st_handle* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
 
-ST_API int st_create(const st_options* opt, const st_ext* ext, st_handle** out) {
-    if (!opt || !out) return fail(nullptr, ST_E_ARG, "st_create: null argument");
-    *out = nullptr;
-    if (opt->numParticles <= 0)
-        return fail(nullptr, ST_E_ARG, "numParticles must be > 0 (got %d)", opt->numParticles);
-    if (opt->integratorType > 1 || opt->integratorType < -1)
-        return fail(nullptr, ST_E_UNSUPPORTED,
-                    "integratorType %d is not implemented (0 = DOP853, 1 = RK45, -1 = trajectory only)",
-                    opt->integratorType);
-    st_ext x;
+static int check_ext(const st_ext* ext, st_ext& x) {
     memset(&x, 0, sizeof x);
     x.pulse_axis = 2;
     if (ext) x = *ext;
@@ -417,12 +465,27 @@ ST_API int st_create(const st_options* opt, const st_ext* ext, st_handle** out) 
         return fail(nullptr, ST_E_ARG, "unknown variant %d", x.variant);
     if (x.engine != ST_ENGINE_PIPELINE && x.engine != ST_ENGINE_FUSED)
         return fail(nullptr, ST_E_ARG, "unknown engine %d", x.engine);
+    return ST_OK;
+}
+
+static int check_options(const st_options& o) {
+    if (o.numParticles <= 0) return fail(nullptr, ST_E_ARG, "numParticles must be > 0 (got %d)", o.numParticles);
+    if (o.integratorType > 1 || o.integratorType < -1)
+        return fail(nullptr, ST_E_UNSUPPORTED,
+                    "integratorType %d is not implemented (0 = DOP853, 1 = RK45, -1 = trajectory only)",
+                    o.integratorType);
+    return ST_OK;
+}
+
+// Allocates the device state of n particles; opt is the handle's (run-wide) options.
+static int create_common(const st_options& opt, const st_ext& x, uint32_t n, st_handle** out) {
     st_handle* h = new (std::nothrow) st_handle();
     if (!h) return fail(nullptr, ST_E_ARG, "out of host memory");
-    h->opt = *opt;
+    h->opt = opt;
+    h->opt.numParticles = (int32_t)n;
     h->ext = x;
-    h->n = (uint32_t)opt->numParticles;
-    h->P = derive_params(*opt, x);
+    h->n = n;
+    h->P = derive_params(opt, x);
     h->state_bytes = (size_t)field_stride(h->n) * F_COUNT;
     hipError_t e = hipMalloc(&h->state, h->state_bytes);
     if (e == hipSuccess) e = hipMalloc(&h->d_work, N_WORK * sizeof(unsigned long long));
@@ -446,11 +509,96 @@ ST_API int st_create(const st_options* opt, const st_ext* ext, st_handle** out) 
     return ST_OK;
 }
 
+ST_API int st_create(const st_options* opt, const st_ext* ext, st_handle** out) {
+    if (!opt || !out) return fail(nullptr, ST_E_ARG, "st_create: null argument");
+    *out = nullptr;
+    int r = check_options(*opt);
+    if (r) return r;
+    st_ext x;
+    if ((r = check_ext(ext, x))) return r;
+    return create_common(*opt, x, (uint32_t)opt->numParticles, out);
+}
+
+// The options that every configuration of a batch must share: geometry,
+// integrator control, time axis and the switches that select kernel code.
+static bool same_run(const st_options& a, const st_options& b, const char** what) {
+#define ST_SAME(f)                 \
+    if (memcmp(&a.f, &b.f, sizeof a.f) != 0) { \
+        *what = #f;                \
+        return false;              \
+    }
+    ST_SAME(L) ST_SAME(t0) ST_SAME(tf) ST_SAME(rtol) ST_SAME(atol) ST_SAME(beta) ST_SAME(safe) ST_SAME(fac1)
+    ST_SAME(fac2) ST_SAME(hmax) ST_SAME(hmin) ST_SAME(h) ST_SAME(maxPosStep) ST_SAME(ioutInt) ST_SAME(nmax)
+    ST_SAME(integratorType) ST_SAME(gas_coll) ST_SAME(gravity) ST_SAME(fixedStepSize) ST_SAME(keepStepSize)
+#undef ST_SAME
+    return true;
+}
+
+ST_API int st_create_batch(const st_options* cfgs, int32_t n_cfg, const st_ext* ext, st_handle** out) {
+    if (!cfgs || !out || n_cfg <= 0) return fail(nullptr, ST_E_ARG, "st_create_batch: null argument or n_cfg <= 0");
+    *out = nullptr;
+    st_ext x;
+    int r = check_ext(ext, x);
+    if (r) return r;
+    if (x.engine != ST_ENGINE_PIPELINE)
+        return fail(nullptr, ST_E_UNSUPPORTED, "batched handles run on the pipeline engine only");
+    uint64_t total = 0;
+    std::vector<uint32_t> off(n_cfg + 1, 0);
+    std::vector<uint32_t> seeds(n_cfg);
+    std::vector<Phys> phys(n_cfg);
+    for (int c = 0; c < n_cfg; ++c) {
+        if ((r = check_options(cfgs[c]))) return r;
+        const char* what = "";
+        if (!same_run(cfgs[0], cfgs[c], &what))
+            return fail(nullptr, ST_E_ARG, "configuration %d differs from configuration 0 in run-wide option '%s'",
+                        c, what);
+        off[c] = (uint32_t)total;
+        total += (uint64_t)cfgs[c].numParticles;
+        seeds[c] = cfgs[c].seed;
+        phys[c] = derive_phys(cfgs[c], x);
+    }
+    if (total > 0x7fffffffull) return fail(nullptr, ST_E_ARG, "too many particles in the batch");
+    off[n_cfg] = (uint32_t)total;
+    st_handle* h = nullptr;
+    if ((r = create_common(cfgs[0], x, (uint32_t)total, &h))) return r;
+    h->batch = true;
+    h->n_cfg = (uint32_t)n_cfg;
+    h->cfg_off = off;
+    hipError_t e = hipMalloc(&h->d_phys, sizeof(Phys) * n_cfg);
+    if (e == hipSuccess) e = hipMalloc(&h->d_cfg_off, sizeof(uint32_t) * (n_cfg + 1));
+    if (e == hipSuccess) e = hipMalloc(&h->d_cfg_seed, sizeof(uint32_t) * n_cfg);
+    if (e == hipSuccess) e = hipMemcpy(h->d_phys, phys.data(), sizeof(Phys) * n_cfg, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(h->d_cfg_off, off.data(), sizeof(uint32_t) * (n_cfg + 1), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(h->d_cfg_seed, seeds.data(), sizeof(uint32_t) * n_cfg, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        fail(nullptr, ST_E_HIP, "batch set-up failed: %s", hipGetErrorString(e));
+        st_destroy(h);
+        return ST_E_HIP;
+    }
+    *out = h;
+    return ST_OK;
+}
+
+ST_API int st_batch_layout(const st_handle* h, int32_t* n_cfg, uint32_t* offsets) {
+    if (!h) return fail(nullptr, ST_E_ARG, "null handle");
+    if (n_cfg) *n_cfg = (int32_t)h->n_cfg;
+    if (offsets) {
+        if (h->batch)
+            memcpy(offsets, h->cfg_off.data(), sizeof(uint32_t) * (h->n_cfg + 1));
+        else {
+            offsets[0] = 0;
+            offsets[1] = h->n;
+        }
+    }
+    return ST_OK;
+}
+
 ST_API void st_destroy(st_handle* h) {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     void* bufs[] = {h->state, h->d_targets, h->d_stop_at, h->d_work, h->d_small, h->d_trace, h->d_trace_count,
-                    h->d_recs, h->d_list[0], h->d_list[1], h->d_seg_count, h->d_backup, h->d_fail};
+                    h->d_recs, h->d_list[0], h->d_list[1], h->d_seg_count, h->d_backup, h->d_fail,
+                    h->d_phys, h->d_cfg_off, h->d_cfg_seed};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : h->ev)
@@ -470,9 +618,15 @@ ST_API int st_set_stream(st_handle* h, void* s) {
 ST_API int st_init_subset(st_handle* h, uint32_t first, uint32_t stride) {
     if (!h) return fail(nullptr, ST_E_ARG, "null handle");
     const st_options& o = h->opt;
+    if (stride == 0) return fail(h, ST_E_ARG, "stride must be > 0");
     const int bs = 256;
-    hipLaunchKernelGGL(k_init, dim3((h->n + bs - 1) / bs), dim3(bs), 0, h->stream, h->P, h->D, h->n, o.seed, first,
-                       stride, o.V, o.t0, d3{o.yi.x, o.yi.y, o.yi.z}, o.tf, (int32_t)(unsigned char)o.dist);
+    const dim3 grid((h->n + bs - 1) / bs);
+    if (h->batch)
+        hipLaunchKernelGGL(k_init<true>, grid, dim3(bs), 0, h->stream, h->P, h->D, h->n, o.seed, first, stride, o.t0,
+                           o.tf, h->d_phys, h->d_cfg_off, h->d_cfg_seed, h->n_cfg);
+    else
+        hipLaunchKernelGGL(k_init<false>, grid, dim3(bs), 0, h->stream, h->P, h->D, h->n, o.seed, first, stride, o.t0,
+                           o.tf, nullptr, nullptr, nullptr, 1u);
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipMemsetAsync(h->d_work, 0, N_WORK * sizeof(unsigned long long), h->stream));
     h->initialized = true;

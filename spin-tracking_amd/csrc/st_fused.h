@@ -140,6 +140,7 @@ enum { PH_DONE = 0, PH_OPEN = 1, PH_EVENT = 2, PH_INT = 3, PH_CLOSE = 4 };
 
 template <int VAR, int INTEG, bool PERSIST>
 __global__ __launch_bounds__(256) void k_engine(const SimParams P, const DevState D, const RunArgs R) {
+    const Phys& F = P.ph;
     Lane L;
     DopState dop;
     RkState rk;
@@ -189,8 +190,8 @@ __global__ __launch_bounds__(256) void k_engine(const SimParams P, const DevStat
         if (phase == PH_EVENT) {
             // one pass of the reference's event loop body up to the
             // integrator call (src/particle.cpp:755-758)
-            next_collision(P, L.tf, L.K, L.rng);
-            move_and_collide(P, L.K, L.rng);
+            next_collision(P, F, L.tf, L.K, L.rng);
+            move_and_collide(P, F, L.K, L.rng);
             sanity(P, L.K);
             if (++L.n_events > R.max_events && L.K.failure == 0) {
                 L.K.stop = true;
@@ -200,10 +201,10 @@ __global__ __launch_bounds__(256) void k_engine(const SimParams P, const DevStat
                 seg.t0 = L.K.t_old;
                 seg.p0 = L.K.pos_old;
                 seg.v0 = L.K.v_old;
-                if (VAR == VAR_FAST) poly = make_poly(P, seg);
+                if (VAR == VAR_FAST) poly = make_poly(P, F, seg);
                 if (INTEG == INT_DOP853) {
-                    if (VAR == VAR_EXACT) dop_begin_exact(P, dop, L.S, seg, L.K.t);
-                    else dop_begin_fast(P, dop, L.S, seg, poly, L.K.t);
+                    if (VAR == VAR_EXACT) dop_begin_exact(P, F, dop, L.S, seg, L.K.t);
+                    else dop_begin_fast(P, F, dop, L.S, seg, poly, L.K.t);
                 } else {
                     rk_begin(rk, seg, L.K.t, L.tempH);
                 }
@@ -220,10 +221,10 @@ __global__ __launch_bounds__(256) void k_engine(const SimParams P, const DevStat
         if (phase == PH_INT) {
             int st;
             if (INTEG == INT_DOP853) {
-                if (VAR == VAR_EXACT) st = dop_trial_exact(P, dop, L.S, L.tempH, seg, L.acc, L.rej);
-                else st = dop_trial_fast(P, dop, L.S, L.tempH, seg, poly, L.acc, L.rej);
+                if (VAR == VAR_EXACT) st = dop_trial_exact(P, F, dop, L.S, L.tempH, seg, L.acc, L.rej);
+                else st = dop_trial_fast(P, F, dop, L.S, L.tempH, seg, poly, L.acc, L.rej);
             } else if (INTEG == INT_RK45) {
-                st = rk_trial<VAR>(P, rk, L.S, L.tempH, seg, poly, L.acc, L.rej);
+                st = rk_trial<VAR>(P, F, rk, L.S, L.tempH, seg, poly, L.acc, L.rej);
             } else {
                 st = 1;  // unreachable: without an integrator no lane enters PH_INT
             }

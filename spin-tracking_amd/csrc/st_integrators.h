@@ -107,7 +107,7 @@ struct DopState {
 
 // ---- exact variant --------------------------------------------------------
 // src/integrator.cpp:114-126: call set-up, including the first RHS eval.
-ST_HD __forceinline__ void dop_begin_exact(const SimParams& P, DopState& D, const d3& y,
+ST_HD __forceinline__ void dop_begin_exact(const SimParams& P, const Phys& F, DopState& D, const d3& y,
                                                 const Seg& S, double t_end) {
     D.x = S.t0;
     D.xf = t_end;
@@ -116,7 +116,7 @@ ST_HD __forceinline__ void dop_begin_exact(const SimParams& P, DopState& D, cons
     D.last = 0;
     D.reject = 0;
     D.nstep = 0;
-    D.k1 = rhs_exact(P, D.x, y, S);
+    D.k1 = rhs_exact(P, F, D.x, y, S);
 }
 
 #define ST_V3(dst, EX, EY, EZ) do { (dst).x = (EX); (dst).y = (EY); (dst).z = (EZ); } while (0)
@@ -124,7 +124,7 @@ ST_HD __forceinline__ void dop_begin_exact(const SimParams& P, DopState& D, cons
 // One iteration of the reference's while(1) (src/integrator.cpp:128-247).
 // Returns 0 to continue, 1 when the call returns 1 (segment done), -1 when
 // it returns -1 (nstep > nmax).  h is the caller's step size (tempH).
-ST_HD inline int dop_trial_exact(const SimParams& P, DopState& D, d3& y, double& h,
+ST_HD inline int dop_trial_exact(const SimParams& P, const Phys& F, DopState& D, d3& y, double& h,
                                       const Seg& S, uint32_t& nacc, uint32_t& nrej) {
     using namespace dop;
     if (D.nstep > P.nmax) return -1;
@@ -141,49 +141,49 @@ ST_HD inline int dop_trial_exact(const SimParams& P, DopState& D, d3& y, double&
         const double ha = h * a21;
         ST_V3(yy, y.x + k1.x * ha, y.y + k1.y * ha, y.z + k1.z * ha);
     }
-    k2 = rhs_exact(P, x + c2 * h, yy, S);
+    k2 = rhs_exact(P, F, x + c2 * h, yy, S);
     ST_V3(yy, y.x + h * (a31 * k1.x + a32 * k2.x), y.y + h * (a31 * k1.y + a32 * k2.y),
           y.z + h * (a31 * k1.z + a32 * k2.z));
-    k3 = rhs_exact(P, x + c3 * h, yy, S);
+    k3 = rhs_exact(P, F, x + c3 * h, yy, S);
     ST_V3(yy, y.x + h * (a41 * k1.x + a43 * k3.x), y.y + h * (a41 * k1.y + a43 * k3.y),
           y.z + h * (a41 * k1.z + a43 * k3.z));
-    k4 = rhs_exact(P, x + c4 * h, yy, S);
+    k4 = rhs_exact(P, F, x + c4 * h, yy, S);
     ST_V3(yy, y.x + h * (a51 * k1.x + a53 * k3.x + a54 * k4.x),
           y.y + h * (a51 * k1.y + a53 * k3.y + a54 * k4.y),
           y.z + h * (a51 * k1.z + a53 * k3.z + a54 * k4.z));
-    k5 = rhs_exact(P, x + c5 * h, yy, S);
+    k5 = rhs_exact(P, F, x + c5 * h, yy, S);
     ST_V3(yy, y.x + h * (a61 * k1.x + a64 * k4.x + a65 * k5.x),
           y.y + h * (a61 * k1.y + a64 * k4.y + a65 * k5.y),
           y.z + h * (a61 * k1.z + a64 * k4.z + a65 * k5.z));
-    k6 = rhs_exact(P, x + c6 * h, yy, S);
+    k6 = rhs_exact(P, F, x + c6 * h, yy, S);
     ST_V3(yy, y.x + h * (a71 * k1.x + a74 * k4.x + a75 * k5.x + a76 * k6.x),
           y.y + h * (a71 * k1.y + a74 * k4.y + a75 * k5.y + a76 * k6.y),
           y.z + h * (a71 * k1.z + a74 * k4.z + a75 * k5.z + a76 * k6.z));
-    k7 = rhs_exact(P, x + c7 * h, yy, S);
+    k7 = rhs_exact(P, F, x + c7 * h, yy, S);
     ST_V3(yy, y.x + h * (a81 * k1.x + a84 * k4.x + a85 * k5.x + a86 * k6.x + a87 * k7.x),
           y.y + h * (a81 * k1.y + a84 * k4.y + a85 * k5.y + a86 * k6.y + a87 * k7.y),
           y.z + h * (a81 * k1.z + a84 * k4.z + a85 * k5.z + a86 * k6.z + a87 * k7.z));
-    k8 = rhs_exact(P, x + c8 * h, yy, S);
+    k8 = rhs_exact(P, F, x + c8 * h, yy, S);
     ST_V3(yy, y.x + h * (a91 * k1.x + a94 * k4.x + a95 * k5.x + a96 * k6.x + a97 * k7.x + a98 * k8.x),
           y.y + h * (a91 * k1.y + a94 * k4.y + a95 * k5.y + a96 * k6.y + a97 * k7.y + a98 * k8.y),
           y.z + h * (a91 * k1.z + a94 * k4.z + a95 * k5.z + a96 * k6.z + a97 * k7.z + a98 * k8.z));
-    k9 = rhs_exact(P, x + c9 * h, yy, S);
+    k9 = rhs_exact(P, F, x + c9 * h, yy, S);
     ST_V3(yy,
           y.x + h * (a101 * k1.x + a104 * k4.x + a105 * k5.x + a106 * k6.x + a107 * k7.x + a108 * k8.x + a109 * k9.x),
           y.y + h * (a101 * k1.y + a104 * k4.y + a105 * k5.y + a106 * k6.y + a107 * k7.y + a108 * k8.y + a109 * k9.y),
           y.z + h * (a101 * k1.z + a104 * k4.z + a105 * k5.z + a106 * k6.z + a107 * k7.z + a108 * k8.z + a109 * k9.z));
-    k10 = rhs_exact(P, x + c10 * h, yy, S);
+    k10 = rhs_exact(P, F, x + c10 * h, yy, S);
     ST_V3(yy,
           y.x + h * (a111 * k1.x + a114 * k4.x + a115 * k5.x + a116 * k6.x + a117 * k7.x + a118 * k8.x + a119 * k9.x + a1110 * k10.x),
           y.y + h * (a111 * k1.y + a114 * k4.y + a115 * k5.y + a116 * k6.y + a117 * k7.y + a118 * k8.y + a119 * k9.y + a1110 * k10.y),
           y.z + h * (a111 * k1.z + a114 * k4.z + a115 * k5.z + a116 * k6.z + a117 * k7.z + a118 * k8.z + a119 * k9.z + a1110 * k10.z));
-    k2 = rhs_exact(P, x + c11 * h, yy, S);
+    k2 = rhs_exact(P, F, x + c11 * h, yy, S);
     const double xph = x + h;
     ST_V3(yy,
           y.x + h * (a121 * k1.x + a124 * k4.x + a125 * k5.x + a126 * k6.x + a127 * k7.x + a128 * k8.x + a129 * k9.x + a1210 * k10.x + a1211 * k2.x),
           y.y + h * (a121 * k1.y + a124 * k4.y + a125 * k5.y + a126 * k6.y + a127 * k7.y + a128 * k8.y + a129 * k9.y + a1210 * k10.y + a1211 * k2.y),
           y.z + h * (a121 * k1.z + a124 * k4.z + a125 * k5.z + a126 * k6.z + a127 * k7.z + a128 * k8.z + a129 * k9.z + a1210 * k10.z + a1211 * k2.z));
-    k3 = rhs_exact(P, xph, yy, S);
+    k3 = rhs_exact(P, F, xph, yy, S);
     ST_V3(k4, b1 * k1.x + b6 * k6.x + b7 * k7.x + b8 * k8.x + b9 * k9.x + b10 * k10.x + b11 * k2.x + b12 * k3.x,
           b1 * k1.y + b6 * k6.y + b7 * k7.y + b8 * k8.y + b9 * k9.y + b10 * k10.y + b11 * k2.y + b12 * k3.y,
           b1 * k1.z + b6 * k6.z + b7 * k7.z + b8 * k8.z + b9 * k9.z + b10 * k10.z + b11 * k2.z + b12 * k3.z);
@@ -211,7 +211,7 @@ ST_HD inline int dop_trial_exact(const SimParams& P, DopState& D, d3& y, double&
     if (err <= 1.0 || P.fixed) {
         D.facold = max_d(err, 1.0E-4);
         nacc++;
-        D.k1 = rhs_exact(P, xph, k5, S);
+        D.k1 = rhs_exact(P, F, xph, k5, S);
         y = k5;
         D.x = xph;
         if (D.last) return 1;
@@ -234,7 +234,7 @@ ST_HD inline int dop_trial_exact(const SimParams& P, DopState& D, d3& y, double&
 // into gamma*B so each stage yields K = h*k directly; products contract into
 // FMAs; the six error-norm divisions become multiplies by 1/sk; and with
 // beta == 0 (the reference default) pow(err, 1/8) is three square roots.
-ST_HD __forceinline__ void dop_begin_fast(const SimParams& P, DopState& D, const d3& y,
+ST_HD __forceinline__ void dop_begin_fast(const SimParams& P, const Phys& F, DopState& D, const d3& y,
                                                const Seg& S, const Poly& q, double t_end) {
     D.x = S.t0;
     D.xf = t_end;
@@ -243,24 +243,24 @@ ST_HD __forceinline__ void dop_begin_fast(const SimParams& P, DopState& D, const
     D.last = 0;
     D.reject = 0;
     D.nstep = 0;
-    D.k1 = rhs_fast(P, D.x, y, S.t0, q);
+    D.k1 = rhs_fast(P, F, D.x, y, S.t0, q);
 }
 
-ST_HD __forceinline__ d3 krhs(const SimParams& P, double t, double tau, const d3& y,
+ST_HD __forceinline__ d3 krhs(const SimParams& P, const Phys& F, double t, double tau, const d3& y,
                                    const Poly& hq, double h) {
 #pragma clang fp contract(fast)
     d3 w{hq.w0.x + tau * (hq.w1.x + tau * hq.w2.x), hq.w0.y + tau * (hq.w1.y + tau * hq.w2.y),
          hq.w0.z + tau * (hq.w1.z + tau * hq.w2.z)};
-    if (P.pulse_on) {
-        const double pv = h * (P.gamma * (P.pa * cos(P.pw * t)));
-        if (P.pulse_axis == 0) w.x += pv;
-        else if (P.pulse_axis == 1) w.y += pv;
+    if (F.pulse_on) {
+        const double pv = h * (F.gamma * (F.pa * cos(F.pw * t)));
+        if (F.pulse_axis == 0) w.x += pv;
+        else if (F.pulse_axis == 1) w.y += pv;
         else w.z += pv;
     }
     return cross3(y, w);
 }
 
-ST_HD inline int dop_trial_fast(const SimParams& P, DopState& D, d3& y, double& h,
+ST_HD inline int dop_trial_fast(const SimParams& P, const Phys& F, DopState& D, d3& y, double& h,
                                      const Seg& S, const Poly& q, uint32_t& nacc, uint32_t& nrej) {
 #pragma clang fp contract(fast)
     using namespace dop;
@@ -276,7 +276,7 @@ ST_HD inline int dop_trial_fast(const SimParams& P, DopState& D, d3& y, double& 
     const Poly hq{d3{h * q.w0.x, h * q.w0.y, h * q.w0.z}, d3{h * q.w1.x, h * q.w1.y, h * q.w1.z},
                   d3{h * q.w2.x, h * q.w2.y, h * q.w2.z}};
     const d3 K1{h * D.k1.x, h * D.k1.y, h * D.k1.z};
-#define KR(C, YY) krhs(P, x + (C) * h, tx + (C) * h, (YY), hq, h)
+#define KR(C, YY) krhs(P, F, x + (C) * h, tx + (C) * h, (YY), hq, h)
     d3 yy, K2, K3, K4, K5, K6, K7, K8, K9, K10, K11, K12;
     ST_V3(yy, y.x + a21 * K1.x, y.y + a21 * K1.y, y.z + a21 * K1.z);
     K2 = KR(c2, yy);
@@ -318,7 +318,7 @@ ST_HD inline int dop_trial_fast(const SimParams& P, DopState& D, d3& y, double& 
           y.x + (a121 * K1.x + a124 * K4.x + a125 * K5.x + a126 * K6.x + a127 * K7.x + a128 * K8.x + a129 * K9.x + a1210 * K10.x + a1211 * K11.x),
           y.y + (a121 * K1.y + a124 * K4.y + a125 * K5.y + a126 * K6.y + a127 * K7.y + a128 * K8.y + a129 * K9.y + a1210 * K10.y + a1211 * K11.y),
           y.z + (a121 * K1.z + a124 * K4.z + a125 * K5.z + a126 * K6.z + a127 * K7.z + a128 * K8.z + a129 * K9.z + a1210 * K10.z + a1211 * K11.z));
-    K12 = krhs(P, xph, tx + h, yy, hq, h);
+    K12 = krhs(P, F, xph, tx + h, yy, hq, h);
 #undef KR
     d3 bs;
     ST_V3(bs, b1 * K1.x + b6 * K6.x + b7 * K7.x + b8 * K8.x + b9 * K9.x + b10 * K10.x + b11 * K11.x + b12 * K12.x,
@@ -352,7 +352,7 @@ ST_HD inline int dop_trial_fast(const SimParams& P, DopState& D, d3& y, double& 
     if (err <= 1.0 || P.fixed) {
         D.facold = max_d(err, 1.0E-4);
         nacc++;
-        D.k1 = rhs_fast(P, xph, y1, S.t0, q);
+        D.k1 = rhs_fast(P, F, xph, y1, S.t0, q);
         y = y1;
         D.x = xph;
         if (D.last) return 1;
@@ -411,12 +411,12 @@ ST_HD __forceinline__ void rk_begin(RkState& R, const Seg& S, double t_end, doub
 // continue, 1 when the call returns 0 (done), -1 / -2 for the reference's
 // error returns.
 template <int VAR>
-ST_HD inline int rk_trial(const SimParams& P, RkState& R, d3& y, double& h, const Seg& S,
+ST_HD inline int rk_trial(const SimParams& P, const Phys& Fp, RkState& R, d3& y, double& h, const Seg& S,
                                const Poly& q, uint32_t& nacc, uint32_t& nrej) {
     using namespace rk;
     auto F = [&](double tt, const d3& yy) -> d3 {
-        if (VAR == VAR_EXACT) return rhs_exact(P, tt, yy, S);
-        return rhs_fast(P, tt, yy, S.t0, q);
+        if (VAR == VAR_EXACT) return rhs_exact(P, Fp, tt, yy, S);
+        return rhs_fast(P, Fp, tt, yy, S.t0, q);
     };
     R.nstep++;
     if ((uint32_t)R.nstep > P.nmax) return -2;

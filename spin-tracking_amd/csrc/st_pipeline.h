@@ -54,7 +54,19 @@ struct PipeArgs {
     uint32_t* fail_count;
     uint32_t fail_cap;
     int32_t integ;           // integratorType
+    const Phys* phys;        // batched handles: physics per configuration (F_CFG indexes it)
 };
+
+// The physics of particle pid: the launch's own block for a plain handle,
+// its configuration's entry for a batched one.
+template <bool BATCH>
+__device__ __forceinline__ const Phys& lane_phys(const SimParams& P, const DevState& D, const Phys* table,
+                                                 uint32_t pid) {
+    if constexpr (BATCH)
+        return table[(uint32_t)D.ld<uint64_t>(F_CFG, pid * 8u)];
+    else
+        return P.ph;
+}
 
 // ---------------------------------------------------------------------------
 // kinematic lane state
@@ -149,7 +161,7 @@ __device__ __forceinline__ void put_rec(char* recs, uint32_t C, uint32_t pid, ui
 // One step of the kinematic state machine: opens an interval if needed,
 // runs one event of the reference's loop body, and returns the record that
 // describes it (or an R_EMPTY interval close).  Shared by k_kin and k_fixup.
-__device__ __forceinline__ uint64_t kin_step(const SimParams& P, const RunArgs& R, int32_t integ, KinLane& L,
+__device__ __forceinline__ uint64_t kin_step(const SimParams& P, const Phys& F, const RunArgs& R, int32_t integ, KinLane& L,
                                              double& rt, d3& rp, d3& rv) {
     bool first = false;
     if (!L.open) {
@@ -181,8 +193,8 @@ __device__ __forceinline__ uint64_t kin_step(const SimParams& P, const RunArgs& 
         first = true;
     }
     // src/particle.cpp:755-758
-    next_collision(P, L.tf, L.K, L.rng);
-    move_and_collide(P, L.K, L.rng);
+    next_collision(P, F, L.tf, L.K, L.rng);
+    move_and_collide(P, F, L.K, L.rng);
     sanity(P, L.K);
     if (++L.n_events > R.max_events && L.K.failure == 0) {
         L.K.stop = true;
@@ -226,12 +238,14 @@ __global__ __launch_bounds__(256) void k_begin_run(const DevState D, uint32_t n,
     list[i] = i;
 }
 
+template <bool BATCH>
 __global__ __launch_bounds__(256) void k_kin(const SimParams P, const DevState D, const PipeArgs A) {
     const uint32_t li = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long cnt[2] = {0, 0};   // events, trips
     if (li < A.n_list) {
         KinLane L;
         kin_load(D, A.list[li], L);
+        const Phys& F = lane_phys<BATCH>(P, D, A.phys, L.pid);
         // back up the kinematic state for a possible spin-failure replay
         const uint32_t o = L.pid * 8u;
         const DevState B{A.backup, D.stride};
@@ -241,7 +255,7 @@ __global__ __launch_bounds__(256) void k_kin(const SimParams P, const DevState D
         while (j < A.C && !L.done) {
             double rt;
             d3 rp, rv;
-            const uint64_t fl = kin_step(P, A.R, A.integ, L, rt, rp, rv);
+            const uint64_t fl = kin_step(P, F, A.R, A.integ, L, rt, rp, rv);
             put_rec(A.recs, A.C, L.pid, j, rt, rp, rv, fl);
             ++j;
             cnt[0] += (fl & R_EMPTY) ? 0 : 1;
@@ -259,6 +273,7 @@ __global__ __launch_bounds__(256) void k_kin(const SimParams P, const DevState D
 // ---------------------------------------------------------------------------
 struct SpinLane {
     uint32_t pid, j, nrec;
+    uint32_t cfg;        // batched handles: the particle's configuration
     int32_t k;           // interval index of the next snapshot block
     bool stopped;
     double h, tempH, t_spin;
@@ -266,9 +281,11 @@ struct SpinLane {
     uint32_t acc, rej, seg;
 };
 
+template <bool BATCH>
 __device__ __forceinline__ void spin_load(const DevState& D, const PipeArgs& A, uint32_t pid, SpinLane& L) {
     const uint32_t o = pid * 8u;
     L.pid = pid;
+    L.cfg = BATCH ? (uint32_t)D.ld<uint64_t>(F_CFG, o) : 0u;
     L.j = 0;
     L.nrec = A.seg_count[pid];
     const uint64_t ss = D.ld<uint64_t>(F_SSTATE, o);
@@ -305,10 +322,17 @@ __device__ __forceinline__ void spin_block(const RunArgs& R, const SpinLane& L) 
     if (R.blocks) write_block_spin(R.blocks + (uint64_t)L.k * R.block_bytes, R.n, L.pid, L.S);
 }
 
-template <int VAR, int INTEG>
+template <int VAR, int INTEG, bool BATCH>
 __global__ __launch_bounds__(256) void k_spin(const SimParams P, const DevState D, const PipeArgs A) {
     const RunArgs& R = A.R;
     SpinLane L;
+    // physics of the lane's current particle
+    auto phys = [&]() -> const Phys& {
+        if constexpr (BATCH)
+            return A.phys[L.cfg];
+        else
+            return P.ph;
+    };
     DopState dop;
     RkState rk;
     Seg seg;
@@ -321,7 +345,7 @@ __global__ __launch_bounds__(256) void k_spin(const SimParams P, const DevState 
     {
         const uint32_t q = acquire_index(R.queue);
         if (q < A.n_list) {
-            spin_load(D, A, A.list[q], L);
+            spin_load<BATCH>(D, A, A.list[q], L);
             phase = S_NEXT;
         }
     }
@@ -336,7 +360,7 @@ __global__ __launch_bounds__(256) void k_spin(const SimParams P, const DevState 
                     phase = S_DONE;
                     const uint32_t q = acquire_index(R.queue);
                     if (q < A.n_list) {
-                        spin_load(D, A, A.list[q], L);
+                        spin_load<BATCH>(D, A, A.list[q], L);
                         phase = S_NEXT;
                         continue;
                     }
@@ -350,10 +374,10 @@ __global__ __launch_bounds__(256) void k_spin(const SimParams P, const DevState 
                     seg.t0 = L.t_spin;
                     seg.p0 = d3{rec.px, rec.py, rec.pz};
                     seg.v0 = d3{rec.vx, rec.vy, rec.vz};
-                    if (VAR == VAR_FAST) poly = make_poly(P, seg);
+                    if (VAR == VAR_FAST) poly = make_poly(P, phys(), seg);
                     if (INTEG == INT_DOP853) {
-                        if (VAR == VAR_EXACT) dop_begin_exact(P, dop, L.S, seg, rec.t);
-                        else dop_begin_fast(P, dop, L.S, seg, poly, rec.t);
+                        if (VAR == VAR_EXACT) dop_begin_exact(P, phys(), dop, L.S, seg, rec.t);
+                        else dop_begin_fast(P, phys(), dop, L.S, seg, poly, rec.t);
                     } else {
                         rk_begin(rk, seg, rec.t, L.tempH);
                     }
@@ -380,10 +404,10 @@ __global__ __launch_bounds__(256) void k_spin(const SimParams P, const DevState 
         if (phase == S_INT) {
             int st;
             if (INTEG == INT_DOP853) {
-                if (VAR == VAR_EXACT) st = dop_trial_exact(P, dop, L.S, L.tempH, seg, L.acc, L.rej);
-                else st = dop_trial_fast(P, dop, L.S, L.tempH, seg, poly, L.acc, L.rej);
+                if (VAR == VAR_EXACT) st = dop_trial_exact(P, phys(), dop, L.S, L.tempH, seg, L.acc, L.rej);
+                else st = dop_trial_fast(P, phys(), dop, L.S, L.tempH, seg, poly, L.acc, L.rej);
             } else {
-                st = rk_trial<VAR>(P, rk, L.S, L.tempH, seg, poly, L.acc, L.rej);
+                st = rk_trial<VAR>(P, phys(), rk, L.S, L.tempH, seg, poly, L.acc, L.rej);
             }
             if (st != 0) {
                 if (R.trace && L.pid == R.trace_pid)
@@ -428,6 +452,7 @@ __global__ __launch_bounds__(256) void k_spin(const SimParams P, const DevState 
 // (the failing segment's event included), then apply the reference's stop:
 // failureState = status, t/pos/v/S NaN, and every later snapshot block shows
 // the stopped particle.
+template <bool BATCH>
 __global__ __launch_bounds__(64) void k_fixup(const SimParams P, const DevState D, const PipeArgs A,
                                               uint32_t n_fail) {
     const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
@@ -440,6 +465,7 @@ __global__ __launch_bounds__(64) void k_fixup(const SimParams P, const DevState 
     for (int q = 0; q < N_KIN_FIELDS; ++q) D.st<uint64_t>(KIN_FIELDS[q], o, B.ld<uint64_t>(KIN_FIELDS[q], o));
     KinLane L;
     kin_load(D, pid, L);
+    const Phys& F = lane_phys<BATCH>(P, D, A.phys, pid);
     RunArgs R = A.R;
     R.blocks = nullptr;   // blocks are rewritten below
     int32_t k_at = L.k;
@@ -447,7 +473,7 @@ __global__ __launch_bounds__(64) void k_fixup(const SimParams P, const DevState 
         k_at = L.k;
         double rt;
         d3 rp, rv;
-        (void)kin_step(P, R, A.integ, L, rt, rp, rv);
+        (void)kin_step(P, F, R, A.integ, L, rt, rp, rv);
     }
     // the failing segment belongs to interval k_at; the stop happens there
     const double nan = __builtin_nan("");

@@ -26,6 +26,7 @@ enum Field : int {
     F_TEMPH,      // spin: step size carried between segments of an interval
     F_TSPIN,      // spin: end time of the last integrated segment
     F_NEVENTS,    // kinematics: events in the open interval
+    F_CFG,        // batched handles: configuration index of the particle
     F_COUNT
 };
 

@@ -86,6 +86,8 @@ SIGNATURES = {
     "st_version": (C.c_char_p, []),
     "st_last_error": (C.c_char_p, [_vp]),
     "st_create": (C.c_int, [C.POINTER(Options), C.POINTER(Ext), C.POINTER(_vp)]),
+    "st_create_batch": (C.c_int, [_vp, _i32, C.POINTER(Ext), C.POINTER(_vp)]),
+    "st_batch_layout": (C.c_int, [_vp, C.POINTER(_i32), _vp]),
     "st_destroy": (None, [_vp]),
     "st_set_stream": (C.c_int, [_vp, _vp]),
     "st_init": (C.c_int, [_vp]),

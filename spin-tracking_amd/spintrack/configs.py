@@ -62,3 +62,51 @@ def dressing(n: int = 262144, species: str = "n", tf: float = 2.0, ioutInt: floa
     o.sqrtKT_m = math.sqrt(K_BOLTZMANN * o.T / o.m)
     o.tc = tc
     return o
+
+
+def batched_sweep(n_cfg: int = 64, n_per: int = 16384, seed: int = 2, tf: float = 20.0,
+                  ioutInt: float = 1.0, g_max: float = 5e-8, dur=(1.0, 20.0),
+                  diffuse: float = 0.1) -> tuple[list, "np.ndarray"]:
+    """Field-configuration sweep for one batched launch: ``n_cfg``
+    configurations of ``n_per`` neutrons (5 m/s isotropic, specular or
+    diffuse walls, no gas, gravity, B0 = (3e-6,0,0) T, DOP853 at
+    rtol = atol = 1e-12).  Per configuration:
+
+    * the nine gradient components dB_i/dx_j (rows Gx, Gy, Gz) are uniform in
+      [-g_max, g_max], drawn config-major from stream (seed, 0);
+    * E = (0, 0, +7.5e6) V/m for even, (0, 0, -7.5e6) V/m for odd configurations;
+    * diffuse = ``diffuse`` for the second half of the configurations, 0 for the first;
+    * particles start with S = (0, 1, 0), transverse to B0, so each
+      configuration's precession phase atan2(Sz, Sy) is defined;
+    * configuration c uses seed ``seed + c*n_per``, so particle j of it is
+      global stream seed + c*n_per + j.
+
+    Returns (configurations, stop_at): particle i (global index) stops at
+    dur[0] + (dur[1]-dur[0])*u with u the first draw of stream (seed, 1 + i).
+    """
+    import numpy as np
+
+    from .rng import Streams, uniforms
+
+    u = uniforms(seed, 0, 9 * n_cfg).reshape(n_cfg, 9)
+    G = g_max * (2.0 * u - 1.0)
+    cfgs = []
+    for c in range(n_cfg):
+        o = neutron_cell(n=n_per, tf=tf, ioutInt=ioutInt, seed=seed + c * n_per)
+        set_fields(o, Gx=tuple(G[c, 0:3]), Gy=tuple(G[c, 3:6]), Gz=tuple(G[c, 6:9]),
+                   E=(0.0, 0.0, 7.5e6 if c % 2 == 0 else -7.5e6),
+                   diffuse=diffuse if c >= n_cfg // 2 else 0.0, yi=(0.0, 1.0, 0.0))
+        cfgs.append(o)
+    n = n_cfg * n_per
+    s = Streams(seed, 1 + np.arange(n, dtype=np.uint64))
+    stop_at = cfgs[0].t0 + dur[0] + (dur[1] - dur[0]) * s.uniform()
+    return cfgs, stop_at
+
+
+def dressing_cotracked(n: int = 262144, tf: float = 2.0, ioutInt: float = 0.1, seed: int = 3,
+                       tc: float = 2e-3) -> list:
+    """The spin-dressing workload with neutrons and He-3 co-tracked: two
+    configurations of one batch (species = configuration), each with ``n``
+    particles; the He-3 population continues the neutron population's seed
+    range (seed + n)."""
+    return [dressing(n, "n", tf, ioutInt, seed, tc), dressing(n, "he3", tf, ioutInt, seed + n, tc)]

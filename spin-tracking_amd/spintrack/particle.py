@@ -64,6 +64,33 @@ class Particle:
         check(self.lib.st_create(C.byref(self.opt), C.byref(self.ext), C.byref(h)), None)
         self.h = h
 
+    @classmethod
+    def batch(cls, cfgs, pulse_axis: int = 2, variant: int = L.VARIANT_EXACT, block_size: int = 0,
+              seg_buffer_mb: int = 0) -> "Particle":
+        """Batched handle (st_create_batch): the configurations ``cfgs`` (each
+        an Options block with its own numParticles) back to back in one
+        ensemble.  ``layout()`` gives the particle offsets."""
+        self = cls.__new__(cls)
+        self.lib = L.load()
+        cfgs = list(cfgs)
+        arr = (Options * len(cfgs))(*[Options.from_buffer_copy(bytes(c)) for c in cfgs])
+        self.opt = Options.from_buffer_copy(bytes(cfgs[0]))
+        self.n = int(sum(int(c.numParticles) for c in cfgs))
+        self.opt.numParticles = self.n
+        self.ext = Ext(int(pulse_axis), int(variant), int(block_size), L.ENGINE_PIPELINE, int(seg_buffer_mb))
+        h = C.c_void_p()
+        check(self.lib.st_create_batch(arr, len(cfgs), C.byref(self.ext), C.byref(h)), None)
+        self.h = h
+        return self
+
+    def layout(self) -> np.ndarray:
+        """Particle offsets of the configurations (n_cfg + 1 entries)."""
+        n = C.c_int32()
+        check(self.lib.st_batch_layout(self.h, C.byref(n), None), self.h)
+        off = np.zeros(n.value + 1, np.uint32)
+        check(self.lib.st_batch_layout(self.h, None, _ptr(off)), self.h)
+        return off
+
     # -- lifetime -----------------------------------------------------------
     def close(self) -> None:
         if getattr(self, "h", None):
