@@ -174,6 +174,96 @@ def bench_sweep(args, variant):
             "kernels": kt, "work": wk, "failures": fails}
 
 
+ADJ_PARAMS = list(range(9)) + [11, 12]   # the nine gradient coefficients, E.z, B0.x
+
+
+def bench_adjoint(args, variant):
+    """Differentiation workload: 131,072 neutrons, 50 s, outputs every 0.5 s,
+    loss = <Sx>(50 s) + accumulated precession phase (least-squares fit over
+    the 100 snapshots).  Times the plain forward run, the checkpointed
+    forward run (kinematic checkpoints every 256 segments) and the adjoint
+    pass (HIP events / host clock around st_adjoint); checks the gradient
+    against central finite differences of the forward run (step 1e-3 of
+    each parameter, and Richardson-extrapolated)."""
+    import numpy as np
+    import spintrack as st
+    from spintrack import adjoint as A
+
+    opt = st.configs.adjoint_case(n=args.adjoint_particles)
+    times = np.asarray(st.output_times(opt))
+    kw = dict(variant=variant, chunk_records=256)
+
+    def forward(o, checkpoint):
+        p = st.Particle(o, **kw)
+        p.set_checkpointing(checkpoint)
+        p.initParticles()
+        buf = st._lib.DeviceBuffer(p.block_bytes() * len(times))
+        p.synchronize()
+        p.timer_start()
+        p.run(times, buf.ptr)
+        ms = p.timer_stop()
+        sums = p.block_sums(buf.ptr, len(times))[:, 0, :]
+        return p, buf, ms, sums
+
+    def loss(o):
+        p, buf, _, sums = forward(o, False)
+        buf.free()
+        p.close()
+        return A.phase_loss(sums[:, 1:] / sums[:, :1], times, o.t0)[0]
+
+    # warm-up (kernel loading)
+    wo = st.configs.adjoint_case(n=2048, tf=1.0)
+    r = A.ForwardRun(wo, variant=variant)
+    r.gradient(A.phase_loss(r.run(), r.times, wo.t0)[1])
+    r.close()
+
+    p, buf, fwd_ms, _ = forward(opt, False)
+    buf.free()
+    p.close()
+    p, buf, fwd_ck_ms, sums = forward(opt, True)
+    if np.any(sums[:, 0] != opt.numParticles):
+        raise RuntimeError("particle failures in the differentiation workload")
+    Lval, seeds = A.phase_loss(sums[:, 1:] / sums[:, :1], times, opt.t0)
+    t0 = time.perf_counter()
+    grad = p.adjoint(buf.ptr, seeds)[0]
+    adj_ms = (time.perf_counter() - t0) * 1e3
+    buf.free()
+    p.close()
+    out = {"workload": f"{opt.numParticles} neutrons, 50 s, diffuse 0.01, gradients U[-1e-8,1e-8] T/m, "
+                       "Ez 7.5e6 V/m, B0x 3e-6 T, loss <Sx>(T) + fitted precession phase over 100 snapshots",
+           "loss": Lval, "forward_ms": fwd_ms, "forward_checkpointed_ms": fwd_ck_ms, "adjoint_ms": adj_ms,
+           "adjoint_over_forward": adj_ms / fwd_ms,
+           "gradient": {st._lib.GRAD_PARAMS[i]: float(grad[i]) for i in ADJ_PARAMS},
+           "particle_seconds": opt.numParticles * (times[-1] - opt.t0)}
+    if not args.no_adjoint_fd:
+        # central differences of the adaptive-step forward run at two steps:
+        # 1e-3 of each parameter (the stated check) and 1e-5; the loss carries
+        # a ~1e-12 noise floor from accept/reject decisions that move with
+        # theta, so weakly coupled parameters resolve only to that floor
+        theta = A.params_of(opt)
+        rel = {}
+        for step in (1e-3, 1e-5):
+            r = {}
+            for i in ADJ_PARAMS:
+                d = step * abs(theta[i])
+                tp, tm = theta.copy(), theta.copy()
+                tp[i] += d
+                tm[i] -= d
+                fd = (loss(A.with_params(opt, tp)) - loss(A.with_params(opt, tm))) / (2 * d)
+                r[st._lib.GRAD_PARAMS[i]] = abs(grad[i] - fd) / abs(fd)
+            rel[step] = r
+        out["fd_rel_err_step_1e-3"] = rel[1e-3]
+        out["fd_rel_err_step_1e-5"] = rel[1e-5]
+        best = {k: min(rel[1e-3][k], rel[1e-5][k]) for k in rel[1e-3]}
+        out["fd_rel_err_best"] = best
+        # loss change of the 1e-3 difference: parameters whose change is
+        # within ~1e3x the loss's noise floor (~1e-12 rad) cannot resolve 1e-6
+        out["fd_loss_change_step_1e-3"] = {st._lib.GRAD_PARAMS[i]: 2e-3 * abs(theta[i] * grad[i])
+                                           for i in ADJ_PARAMS}
+        out["fd_params_within_1e-6"] = sum(v < 1e-6 for v in best.values())
+    return out
+
+
 def cpu_baseline(opt, n_total, sample_particles, sample_intervals):
     """The CPU oracle (the reference CPU path restated in C, OpenMP over
     particles) on a deterministic subset: every k-th particle of the same
@@ -216,6 +306,9 @@ def main():
     ap.add_argument("--engine", choices=["pipeline", "fused"], default="pipeline")
     ap.add_argument("--seg-buffer-mb", type=int, default=0, help="pipeline staging budget (0 = library default)")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched field-sweep leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiation leg")
+    ap.add_argument("--no-adjoint-fd", action="store_true", help="skip its finite-difference check")
+    ap.add_argument("--adjoint-particles", type=int, default=131072)
     ap.add_argument("--sweep-configs", type=int, default=64)
     ap.add_argument("--sweep-per", type=int, default=16384, help="neutrons per sweep configuration")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
@@ -283,6 +376,11 @@ def main():
         sweep = bench_sweep(args, variant)
         print(f"[bench] sweep: {sweep}", file=sys.stderr, flush=True)
 
+    adj = None
+    if not args.no_adjoint:
+        adj = bench_adjoint(args, variant)
+        print(f"[bench] adjoint: {adj}", file=sys.stderr, flush=True)
+
     line = {
         "metric": "particle-seconds/s, 1 MI355X FP64 rtol=1e-12",
         "value": value,
@@ -309,6 +407,7 @@ def main():
         "cpu_baseline": cpu,
         "failures": res["failures"],
         "batched_sweep": sweep,
+        "adjoint": adj,
     }
     print(json.dumps(line))
 

@@ -73,7 +73,9 @@ typedef struct st_ext {
     int32_t block_size;     /* threads per workgroup of the fused plain kernel; 0 = 256 */
     int32_t engine;         /* ST_ENGINE_* */
     int32_t seg_buffer_mb;  /* HBM budget of the pipeline's segment-record buffer; 0 = 16384 */
-    int32_t reserved[3];
+    int32_t chunk_records;  /* trajectory segments per particle per pipeline chunk (and between
+                               adjoint checkpoints); 0 = as many as seg_buffer_mb holds */
+    int32_t reserved[2];
 } st_ext;
 
 /* include/particle.h:52-59 */
@@ -226,6 +228,33 @@ ST_API int st_timer_stop(st_handle* h, float* ms);
  * counts ([0] kinematics, [1] spin).  Enabling resets the sums. */
 ST_API int st_kernel_timing(st_handle* h, int enable);
 ST_API int st_kernel_times(st_handle* h, double ms[2], uint64_t launches[2]);
+
+/* Differentiation (no counterpart in the reference).  With checkpointing on,
+ * each later st_advance_to / st_run keeps the kinematic state of every
+ * particle at each pipeline chunk start, i.e. every st_ext.chunk_records
+ * trajectory segments; a new run replaces the previous run's checkpoints. */
+ST_API int st_set_checkpointing(st_handle* h, int enable);
+
+/* Adjoint pass over the last checkpointed st_run(n_targets, targets,
+ * blocks_dev), which must have run without particle failures.  For a loss
+ * L that depends on the spins only through the ensemble means <S>(t_k) at
+ * the output times, seeds[(c*n_targets + k)*3 + i] = dL/d<S_i>(t_k) of
+ * configuration c (one configuration for a plain handle).  The kernel
+ * integrates u = lambda x S backwards along the replayed trajectories
+ * (DOP853, the run's rtol/atol) and returns, per configuration,
+ * grad[c*15 + f] = dL/dtheta_f for theta = (Gx.x, Gx.y, Gx.z, Gy.x, Gy.y,
+ * Gy.z, Gz.x, Gz.y, Gz.z, E.x, E.y, E.z, B0.x, B0.y, B0.z), i.e. the
+ * gradient coefficients dB_i/dx_j (row i), E and B0.  blocks_dev holds the
+ * run's snapshot blocks (their S arrays give S(t_k)). */
+ST_API int st_adjoint(st_handle* h, const void* blocks_dev, const double* seeds, double* grad);
+
+/* Ensemble sums over n_blocks consecutive snapshot blocks in device memory
+ * (on-device form of the reference's analysis selection,
+ * benchmarks/CheckingFrequencyShiftFromEField.ipynb:29-35): for block k and
+ * configuration c, out[(k*n_cfg + c)*4 + {0,1,2,3}] = number of particles
+ * with failure == 0 and finite S, and the sums of Sx, Sy, Sz over them.
+ * Deterministic (fixed-order partial sums).  Waits for the result. */
+ST_API int st_block_sums(st_handle* h, const void* blocks_dev, int32_t n_blocks, double* out);
 
 /* Waits for all work queued on the handle's stream. */
 ST_API int st_synchronize(st_handle* h);

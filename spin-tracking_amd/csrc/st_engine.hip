@@ -28,6 +28,8 @@
 #include <vector>
 
 #include "../../include/spintrack.h"
+#include "st_adjoint.h"
+#include "st_analysis.h"
 #include "st_fused.h"
 #include "st_pipeline.h"
 
@@ -188,6 +190,21 @@ struct st_handle {
     Phys* d_phys = nullptr;
     uint32_t* d_cfg_off = nullptr;
     uint32_t* d_cfg_seed = nullptr;
+    // adjoint checkpoints of the last pipeline run: kinematic state at each
+    // chunk start and that chunk's active list
+    struct Ckpt {
+        char* state;
+        uint32_t* list;
+        uint32_t n_list;
+    };
+    bool ckpt_on = false;
+    std::vector<Ckpt> ckpts;
+    RunArgs ckpt_run;               // the run the checkpoints belong to
+    double* d_ckpt_targets = nullptr;
+    // adjoint work buffers
+    char* d_adj = nullptr;          // A_COUNT fields
+    char* d_replay = nullptr;       // kinematic state of the replayed chunk
+    char* d_scratch = nullptr;      // next list, counters, failure list of replays
     std::string err;
 };
 
@@ -346,6 +363,7 @@ static int ensure_pipeline(st_handle* h) {
     if (h->d_recs) return ST_OK;
     const uint64_t budget = (uint64_t)(h->ext.seg_buffer_mb > 0 ? h->ext.seg_buffer_mb : 16384) << 20;
     uint64_t C = budget / ((uint64_t)h->n * sizeof(SegRec));
+    if (h->ext.chunk_records > 0) C = (uint64_t)h->ext.chunk_records;
     C = std::min<uint64_t>(std::max<uint64_t>(C, 16), 1u << 20);
     h->rec_cap = (uint32_t)C;
     HIPCHK(h, hipMalloc(&h->d_recs, (size_t)C * h->n * sizeof(SegRec)));
@@ -357,9 +375,38 @@ static int ensure_pipeline(st_handle* h) {
     return ST_OK;
 }
 
+static void free_ckpts(st_handle* h) {
+    for (auto& c : h->ckpts) {
+        if (c.state) (void)hipFree(c.state);
+        if (c.list) (void)hipFree(c.list);
+    }
+    h->ckpts.clear();
+}
+
+// Keeps the kinematic state at this chunk's start (k_kin has just backed it
+// up for the chunk's particles) and the chunk's active list.
+static int save_ckpt(st_handle* h, const uint32_t* list, uint32_t n_list) {
+    st_handle::Ckpt c{nullptr, nullptr, n_list};
+    HIPCHK(h, hipMalloc(&c.state, h->state_bytes));
+    hipError_t e = hipMalloc(&c.list, sizeof(uint32_t) * n_list);
+    if (e != hipSuccess) {
+        (void)hipFree(c.state);
+        return fail(h, ST_E_HIP, "checkpoint allocation failed: %s", hipGetErrorString(e));
+    }
+    h->ckpts.push_back(c);
+    HIPCHK(h, hipMemcpyAsync(c.state, h->d_backup, h->state_bytes, hipMemcpyDeviceToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(c.list, list, sizeof(uint32_t) * n_list, hipMemcpyDeviceToDevice, h->stream));
+    return ST_OK;
+}
+
 static int run_pipeline(st_handle* h, RunArgs R) {
     int r = ensure_pipeline(h);
     if (r) return r;
+    if (!h->ckpts.empty()) {
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        free_ckpts(h);
+    }
+    h->ckpt_run = R;
     const int integ = h->opt.integratorType;
     if (integ != INT_NONE && (r = persistent_grid(h, spin_symbol(h->ext.variant, integ, h->batch), &h->grid_spin)))
         return r;
@@ -394,6 +441,7 @@ static int run_pipeline(st_handle* h, RunArgs R) {
             hipLaunchKernelGGL(k_kin<false>, dim3((n_list + bs - 1) / bs), dim3(bs), 0, h->stream, h->P, h->D, A);
         HIPCHK(h, hipGetLastError());
         if (h->ktime_on) HIPCHK(h, hipEventRecord(h->kev[1], h->stream));
+        if (h->ckpt_on && (r = save_ckpt(h, A.list, n_list))) return r;
         if (spin) {
             // the spin kernel only needs its queue head reset; k_kin left the
             // counts in d_small[1..2] and does not touch d_small[0]
@@ -598,7 +646,8 @@ ST_API void st_destroy(st_handle* h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     void* bufs[] = {h->state, h->d_targets, h->d_stop_at, h->d_work, h->d_small, h->d_trace, h->d_trace_count,
                     h->d_recs, h->d_list[0], h->d_list[1], h->d_seg_count, h->d_backup, h->d_fail,
-                    h->d_phys, h->d_cfg_off, h->d_cfg_seed};
+                    h->d_phys, h->d_cfg_off, h->d_cfg_seed, h->d_adj, h->d_replay, h->d_scratch};
+    free_ckpts(h);
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     for (hipEvent_t e : h->ev)
@@ -684,6 +733,183 @@ ST_API int st_run(st_handle* h, int32_t n_targets, const double* targets, void* 
     const int r = h->ext.engine == ST_ENGINE_FUSED ? run_fused(h, R, true) : run_pipeline(h, R);
     if (r) return r;
     h->last_target = targets[n_targets - 1];
+    return ST_OK;
+}
+
+ST_API int st_set_checkpointing(st_handle* h, int enable) {
+    if (!h) return fail(nullptr, ST_E_ARG, "null handle");
+    if (enable && h->ext.engine != ST_ENGINE_PIPELINE)
+        return fail(h, ST_E_UNSUPPORTED, "checkpointing needs the pipeline engine");
+    h->ckpt_on = enable != 0;
+    return ST_OK;
+}
+
+ST_API int st_adjoint(st_handle* h, const void* blocks_dev, const double* seeds, double* grad) {
+    if (!h || !blocks_dev || !seeds || !grad) return fail(h, ST_E_ARG, "null argument");
+    if (h->ckpts.empty()) return fail(h, ST_E_STATE, "st_adjoint needs a checkpointed pipeline run first");
+    const RunArgs& R0 = h->ckpt_run;
+    const uint32_t K = (uint32_t)R0.n_targets;
+    for (size_t i = 0; i < (size_t)3 * K * h->n_cfg; ++i)
+        if (!isfinite(seeds[i])) return fail(h, ST_E_ARG, "seed %zu is not finite", i);
+    const int bs = 256;
+    if (!h->d_adj) HIPCHK(h, hipMalloc(&h->d_adj, (size_t)field_stride(h->n) * A_COUNT));
+    if (!h->d_replay) HIPCHK(h, hipMalloc(&h->d_replay, h->state_bytes));
+    const size_t scratch_bytes = 256 + sizeof(uint32_t) * (size_t)h->n * 4 + sizeof(double) * 3 * K * h->n_cfg;
+    if (h->d_scratch) HIPCHK(h, hipFree(h->d_scratch));
+    h->d_scratch = nullptr;
+    HIPCHK(h, hipMalloc(&h->d_scratch, scratch_bytes));
+    char* sp = h->d_scratch;
+    unsigned long long* work = (unsigned long long*)sp;    // 8 counters
+    uint32_t* small = (uint32_t*)(sp + 64);                // queue, next_count, fail_count
+    uint32_t* next_list = (uint32_t*)(sp + 256);
+    uint32_t* fail_list = next_list + h->n;                // 3 per particle
+    double* d_seeds = (double*)(fail_list + 3 * (size_t)h->n);
+    HIPCHK(h, hipMemcpyAsync(d_seeds, seeds, sizeof(double) * 3 * K * h->n_cfg, hipMemcpyHostToDevice, h->stream));
+    const DevState AD{h->d_adj, field_stride(h->n)};
+    const DevState DR{h->d_replay, h->D.stride};
+    hipLaunchKernelGGL(k_adj_begin, dim3((h->n + bs - 1) / bs), dim3(bs), 0, h->stream, AD, h->n, (int32_t)K - 1,
+                       h->P.h0);
+    HIPCHK(h, hipGetLastError());
+    if (h->batch)   // the replay reads each particle's configuration from its state
+        HIPCHK(h, hipMemcpyAsync(DR.f<char>(F_CFG), h->D.f<char>(F_CFG), sizeof(uint64_t) * h->n,
+                                 hipMemcpyDeviceToDevice, h->stream));
+    PipeArgs A;
+    memset(&A, 0, sizeof A);
+    A.R = R0;
+    A.R.blocks = nullptr;     // the replay rewrites no snapshot block
+    A.R.work = work;
+    A.R.queue = small;
+    A.R.trace = nullptr;
+    A.recs = h->d_recs;
+    A.C = h->rec_cap;
+    A.seg_count = h->d_seg_count;
+    A.backup = h->d_backup;
+    A.fail_list = fail_list;
+    A.fail_cap = h->n;
+    A.fail_count = small + 2;
+    A.next_count = small + 1;
+    A.next_list = next_list;
+    A.integ = h->opt.integratorType;
+    A.phys = h->d_phys;
+    AdjArgs J;
+    memset(&J, 0, sizeof J);
+    J.recs = h->d_recs;
+    J.C = h->rec_cap;
+    J.seg_count = h->d_seg_count;
+    J.adj = AD;
+    J.blocks = (const char*)blocks_dev;
+    J.block_bytes = (uint64_t)108 * h->n;
+    J.n = h->n;
+    J.seeds = d_seeds;
+    J.n_targets = K;
+    J.phys = h->d_phys;
+    for (size_t m = h->ckpts.size(); m-- > 0;) {
+        const st_handle::Ckpt& c = h->ckpts[m];
+        // replay chunk m's kinematics from its start state: the records are
+        // bit-identical to the forward run's
+        for (int f : KIN_FIELDS)
+            HIPCHK(h, hipMemcpyAsync(DR.f<char>(f), c.state + (size_t)f * h->D.stride, h->D.stride,
+                                     hipMemcpyDeviceToDevice, h->stream));
+        HIPCHK(h, hipMemsetAsync(small, 0, 4 * sizeof(uint32_t), h->stream));
+        A.list = c.list;
+        A.n_list = c.n_list;
+        const dim3 grid((c.n_list + bs - 1) / bs);
+        if (h->batch)
+            hipLaunchKernelGGL(k_kin<true>, grid, dim3(bs), 0, h->stream, h->P, DR, A);
+        else
+            hipLaunchKernelGGL(k_kin<false>, grid, dim3(bs), 0, h->stream, h->P, DR, A);
+        HIPCHK(h, hipGetLastError());
+        J.list = c.list;
+        J.n_list = c.n_list;
+        J.ck = DevState{c.state, h->D.stride};
+        if (h->batch)
+            hipLaunchKernelGGL(k_adj<true>, grid, dim3(bs), 0, h->stream, h->P, h->D, J);
+        else
+            hipLaunchKernelGGL(k_adj<false>, grid, dim3(bs), 0, h->stream, h->P, h->D, J);
+        HIPCHK(h, hipGetLastError());
+    }
+    // deterministic reduction: per-block partials per configuration, summed
+    // on the host in block order
+    uint32_t max_cfg = 0;
+    for (uint32_t c = 0; c < h->n_cfg; ++c) {
+        const uint32_t lo = h->batch ? h->cfg_off[c] : 0, hi = h->batch ? h->cfg_off[c + 1] : h->n;
+        max_cfg = std::max(max_cfg, hi - lo);
+    }
+    const uint32_t bpc = std::max(1u, std::min<uint32_t>(1024, (max_cfg + 255) / 256));
+    double* d_part = nullptr;
+    uint32_t* d_off = nullptr;
+    std::vector<uint32_t> off = h->batch ? h->cfg_off : std::vector<uint32_t>{0, h->n};
+    HIPCHK(h, hipMalloc(&d_part, sizeof(double) * N_GRAD * bpc * h->n_cfg));
+    hipError_t e = hipMalloc(&d_off, sizeof(uint32_t) * off.size());
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_off, off.data(), sizeof(uint32_t) * off.size(), hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_adj_partials, dim3(bpc, h->n_cfg), dim3(256), 0, h->stream, AD, d_off, bpc, d_part);
+        e = hipGetLastError();
+    }
+    std::vector<double> part((size_t)N_GRAD * bpc * h->n_cfg);
+    std::vector<int64_t> fails(h->n);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(part.data(), d_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(fails.data(), AD.f<char>(A_FAIL), sizeof(int64_t) * h->n, hipMemcpyDeviceToHost,
+                           h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(d_part);
+    (void)hipFree(d_off);
+    if (e != hipSuccess) return fail(h, ST_E_HIP, "adjoint reduction failed: %s", hipGetErrorString(e));
+    for (uint32_t i = 0; i < h->n; ++i)
+        if (fails[i]) return fail(h, ST_E_STATE, "adjoint integration failed for particle %u (status %lld)", i,
+                                  (long long)fails[i]);
+    // gradient = gamma/N * (int u_i r_j, int u x v / c^2, int u)
+    std::vector<Phys> phys(h->n_cfg);
+    if (h->batch)
+        HIPCHK(h, hipMemcpy(phys.data(), h->d_phys, sizeof(Phys) * h->n_cfg, hipMemcpyDeviceToHost));
+    else
+        phys[0] = h->P.ph;
+    for (uint32_t c = 0; c < h->n_cfg; ++c) {
+        double s[N_GRAD] = {0};
+        for (uint32_t b = 0; b < bpc; ++b)
+            for (int f = 0; f < N_GRAD; ++f) s[f] += part[((size_t)c * bpc + b) * N_GRAD + f];
+        const double nc = (double)(off[c + 1] - off[c]);
+        const double gm = phys[c].gamma / nc;
+        for (int f = 0; f < N_GRAD; ++f) grad[c * N_GRAD + f] = gm * s[f] * ((f >= 9 && f < 12) ? IC2 : 1.0);
+    }
+    return ST_OK;
+}
+
+ST_API int st_block_sums(st_handle* h, const void* blocks_dev, int32_t n_blocks, double* out) {
+    if (!h || !blocks_dev || !out || n_blocks <= 0) return fail(h, ST_E_ARG, "null argument or n_blocks <= 0");
+    std::vector<uint32_t> off = h->batch ? h->cfg_off : std::vector<uint32_t>{0, h->n};
+    uint32_t max_cfg = 0;
+    for (uint32_t c = 0; c < h->n_cfg; ++c) max_cfg = std::max(max_cfg, off[c + 1] - off[c]);
+    const uint32_t wpc = std::max(1u, std::min<uint32_t>(256, (max_cfg + 255) / 256));
+    const size_t n_part = (size_t)n_blocks * h->n_cfg * wpc * N_SUMS;
+    double* d_part = nullptr;
+    uint32_t* d_off = nullptr;
+    HIPCHK(h, hipMalloc(&d_part, sizeof(double) * n_part));
+    hipError_t e = hipMalloc(&d_off, sizeof(uint32_t) * off.size());
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_off, off.data(), sizeof(uint32_t) * off.size(), hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_block_sums, dim3(wpc, n_blocks, h->n_cfg), dim3(256), 0, h->stream,
+                           (const char*)blocks_dev, (uint64_t)108 * h->n, h->n, d_off, wpc, d_part);
+        e = hipGetLastError();
+    }
+    std::vector<double> part(n_part);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(part.data(), d_part, sizeof(double) * n_part, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    (void)hipFree(d_part);
+    (void)hipFree(d_off);
+    if (e != hipSuccess) return fail(h, ST_E_HIP, "block sums failed: %s", hipGetErrorString(e));
+    for (int32_t k = 0; k < n_blocks; ++k)
+        for (uint32_t c = 0; c < h->n_cfg; ++c) {
+            double s[N_SUMS] = {0, 0, 0, 0};
+            for (uint32_t w = 0; w < wpc; ++w)
+                for (int f = 0; f < N_SUMS; ++f) s[f] += part[(((size_t)k * h->n_cfg + c) * wpc + w) * N_SUMS + f];
+            for (int f = 0; f < N_SUMS; ++f) out[((size_t)k * h->n_cfg + c) * N_SUMS + f] = s[f];
+        }
     return ST_OK;
 }
 

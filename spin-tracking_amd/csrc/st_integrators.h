@@ -260,8 +260,33 @@ ST_HD __forceinline__ d3 krhs(const SimParams& P, const Phys& F, double t, doubl
     return cross3(y, w);
 }
 
-ST_HD inline int dop_trial_fast(const SimParams& P, const Phys& F, DopState& D, d3& y, double& h,
-                                     const Seg& S, const Poly& q, uint32_t& nacc, uint32_t& nrej) {
+// Quadrature carried along an adjoint (backward) segment: moments
+// N_n = int u(s) s^n ds, n = 0, 1, 2, of the integrated vector over the
+// segment's own time s, by the step's 8th-order weights (b1, b6..b12) at the
+// stage inputs; and the physical time the pulse is evaluated at,
+// t = t_end - s, with the field negated (du/ds = -u x omega(t_end - s)).
+struct AdjQuad {
+    d3 n0, n1, n2;
+    double t_end;
+};
+
+ST_HD __forceinline__ d3 krhs_adj(const Phys& F, double t_phys, double tau, const d3& y, const Poly& hq,
+                                  double h) {
+#pragma clang fp contract(fast)
+    d3 w{hq.w0.x + tau * (hq.w1.x + tau * hq.w2.x), hq.w0.y + tau * (hq.w1.y + tau * hq.w2.y),
+         hq.w0.z + tau * (hq.w1.z + tau * hq.w2.z)};
+    if (F.pulse_on) {
+        const double pv = -(h * (F.gamma * (F.pa * cos(F.pw * t_phys))));
+        if (F.pulse_axis == 0) w.x += pv;
+        else if (F.pulse_axis == 1) w.y += pv;
+        else w.z += pv;
+    }
+    return cross3(y, w);
+}
+
+template <bool ADJ>
+ST_HD inline int dop_trial_fast_t(const SimParams& P, const Phys& F, DopState& D, d3& y, double& h,
+                                  const Seg& S, const Poly& q, uint32_t& nacc, uint32_t& nrej, AdjQuad* Q) {
 #pragma clang fp contract(fast)
     using namespace dop;
     if (D.nstep > P.nmax) return -1;
@@ -276,7 +301,20 @@ ST_HD inline int dop_trial_fast(const SimParams& P, const Phys& F, DopState& D, 
     const Poly hq{d3{h * q.w0.x, h * q.w0.y, h * q.w0.z}, d3{h * q.w1.x, h * q.w1.y, h * q.w1.z},
                   d3{h * q.w2.x, h * q.w2.y, h * q.w2.z}};
     const d3 K1{h * D.k1.x, h * D.k1.y, h * D.k1.z};
-#define KR(C, YY) krhs(P, F, x + (C) * h, tx + (C) * h, (YY), hq, h)
+#define KR(C, YY)                                                                        \
+    (ADJ ? krhs_adj(F, Q->t_end - (x + (C) * h), tx + (C) * h, (YY), hq, h)           \
+         : krhs(P, F, x + (C) * h, tx + (C) * h, (YY), hq, h))
+    // adjoint quadrature of this trial: sum_i b_i Y_i s_i^n
+    d3 q0{0.0, 0.0, 0.0}, q1{0.0, 0.0, 0.0}, q2{0.0, 0.0, 0.0};
+    auto quad = [&](double b, double si, const d3& Y) {
+        if constexpr (ADJ) {
+            const double w1 = b * si, w2 = w1 * si;
+            q0 = d3{q0.x + b * Y.x, q0.y + b * Y.y, q0.z + b * Y.z};
+            q1 = d3{q1.x + w1 * Y.x, q1.y + w1 * Y.y, q1.z + w1 * Y.z};
+            q2 = d3{q2.x + w2 * Y.x, q2.y + w2 * Y.y, q2.z + w2 * Y.z};
+        }
+    };
+    quad(b1, tx, y);
     d3 yy, K2, K3, K4, K5, K6, K7, K8, K9, K10, K11, K12;
     ST_V3(yy, y.x + a21 * K1.x, y.y + a21 * K1.y, y.z + a21 * K1.z);
     K2 = KR(c2, yy);
@@ -291,34 +329,41 @@ ST_HD inline int dop_trial_fast(const SimParams& P, const Phys& F, DopState& D, 
     K5 = KR(c5, yy);
     ST_V3(yy, y.x + (a61 * K1.x + a64 * K4.x + a65 * K5.x), y.y + (a61 * K1.y + a64 * K4.y + a65 * K5.y),
           y.z + (a61 * K1.z + a64 * K4.z + a65 * K5.z));
+    quad(b6, tx + c6 * h, yy);
     K6 = KR(c6, yy);
     ST_V3(yy, y.x + (a71 * K1.x + a74 * K4.x + a75 * K5.x + a76 * K6.x),
           y.y + (a71 * K1.y + a74 * K4.y + a75 * K5.y + a76 * K6.y),
           y.z + (a71 * K1.z + a74 * K4.z + a75 * K5.z + a76 * K6.z));
+    quad(b7, tx + c7 * h, yy);
     K7 = KR(c7, yy);
     ST_V3(yy, y.x + (a81 * K1.x + a84 * K4.x + a85 * K5.x + a86 * K6.x + a87 * K7.x),
           y.y + (a81 * K1.y + a84 * K4.y + a85 * K5.y + a86 * K6.y + a87 * K7.y),
           y.z + (a81 * K1.z + a84 * K4.z + a85 * K5.z + a86 * K6.z + a87 * K7.z));
+    quad(b8, tx + c8 * h, yy);
     K8 = KR(c8, yy);
     ST_V3(yy, y.x + (a91 * K1.x + a94 * K4.x + a95 * K5.x + a96 * K6.x + a97 * K7.x + a98 * K8.x),
           y.y + (a91 * K1.y + a94 * K4.y + a95 * K5.y + a96 * K6.y + a97 * K7.y + a98 * K8.y),
           y.z + (a91 * K1.z + a94 * K4.z + a95 * K5.z + a96 * K6.z + a97 * K7.z + a98 * K8.z));
+    quad(b9, tx + c9 * h, yy);
     K9 = KR(c9, yy);
     ST_V3(yy, y.x + (a101 * K1.x + a104 * K4.x + a105 * K5.x + a106 * K6.x + a107 * K7.x + a108 * K8.x + a109 * K9.x),
           y.y + (a101 * K1.y + a104 * K4.y + a105 * K5.y + a106 * K6.y + a107 * K7.y + a108 * K8.y + a109 * K9.y),
           y.z + (a101 * K1.z + a104 * K4.z + a105 * K5.z + a106 * K6.z + a107 * K7.z + a108 * K8.z + a109 * K9.z));
+    quad(b10, tx + c10 * h, yy);
     K10 = KR(c10, yy);
     ST_V3(yy,
           y.x + (a111 * K1.x + a114 * K4.x + a115 * K5.x + a116 * K6.x + a117 * K7.x + a118 * K8.x + a119 * K9.x + a1110 * K10.x),
           y.y + (a111 * K1.y + a114 * K4.y + a115 * K5.y + a116 * K6.y + a117 * K7.y + a118 * K8.y + a119 * K9.y + a1110 * K10.y),
           y.z + (a111 * K1.z + a114 * K4.z + a115 * K5.z + a116 * K6.z + a117 * K7.z + a118 * K8.z + a119 * K9.z + a1110 * K10.z));
+    quad(b11, tx + c11 * h, yy);
     K11 = KR(c11, yy);
     const double xph = x + h;
     ST_V3(yy,
           y.x + (a121 * K1.x + a124 * K4.x + a125 * K5.x + a126 * K6.x + a127 * K7.x + a128 * K8.x + a129 * K9.x + a1210 * K10.x + a1211 * K11.x),
           y.y + (a121 * K1.y + a124 * K4.y + a125 * K5.y + a126 * K6.y + a127 * K7.y + a128 * K8.y + a129 * K9.y + a1210 * K10.y + a1211 * K11.y),
           y.z + (a121 * K1.z + a124 * K4.z + a125 * K5.z + a126 * K6.z + a127 * K7.z + a128 * K8.z + a129 * K9.z + a1210 * K10.z + a1211 * K11.z));
-    K12 = krhs(P, F, xph, tx + h, yy, hq, h);
+    quad(b12, tx + h, yy);
+    K12 = ADJ ? krhs_adj(F, Q->t_end - xph, tx + h, yy, hq, h) : krhs(P, F, xph, tx + h, yy, hq, h);
 #undef KR
     d3 bs;
     ST_V3(bs, b1 * K1.x + b6 * K6.x + b7 * K7.x + b8 * K8.x + b9 * K9.x + b10 * K10.x + b11 * K11.x + b12 * K12.x,
@@ -352,7 +397,16 @@ ST_HD inline int dop_trial_fast(const SimParams& P, const Phys& F, DopState& D, 
     if (err <= 1.0 || P.fixed) {
         D.facold = max_d(err, 1.0E-4);
         nacc++;
-        D.k1 = rhs_fast(P, F, xph, y1, S.t0, q);
+        if constexpr (ADJ) {
+            // derivative at the new point for the next trial (FSAL), and the
+            // accepted step's quadrature
+            D.k1 = krhs_adj(F, Q->t_end - xph, tx + h, y1, q, 1.0);
+            Q->n0 = d3{Q->n0.x + h * q0.x, Q->n0.y + h * q0.y, Q->n0.z + h * q0.z};
+            Q->n1 = d3{Q->n1.x + h * q1.x, Q->n1.y + h * q1.y, Q->n1.z + h * q1.z};
+            Q->n2 = d3{Q->n2.x + h * q2.x, Q->n2.y + h * q2.y, Q->n2.z + h * q2.z};
+        } else {
+            D.k1 = rhs_fast(P, F, xph, y1, S.t0, q);
+        }
         y = y1;
         D.x = xph;
         if (D.last) return 1;
@@ -367,6 +421,11 @@ ST_HD inline int dop_trial_fast(const SimParams& P, const Phys& F, DopState& D, 
     }
     if (!P.fixed) h = hnew;
     return 0;
+}
+
+ST_HD __forceinline__ int dop_trial_fast(const SimParams& P, const Phys& F, DopState& D, d3& y, double& h,
+                                         const Seg& S, const Poly& q, uint32_t& nacc, uint32_t& nrej) {
+    return dop_trial_fast_t<false>(P, F, D, y, h, S, q, nacc, nrej, nullptr);
 }
 
 // ---------------------------------------------------------------------------

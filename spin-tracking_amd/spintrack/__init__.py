@@ -15,4 +15,4 @@ from .options import (HELIUM3, NEUTRON, default_options, derive, n_intervals,  #
                       output_times, parse_params, read_params, set_fields)
 from .particle import COORDS, Particle, snapshot_block_dtype, xyz  # noqa: F401
 from .logger import BinaryLogger, read_output  # noqa: F401
-from . import configs, simulation  # noqa: F401
+from . import adjoint, configs, rng, simulation  # noqa: F401

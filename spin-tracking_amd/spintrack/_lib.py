@@ -58,7 +58,8 @@ ENGINE_FUSED = 1
 
 class Ext(C.Structure):
     _fields_ = [("pulse_axis", C.c_int32), ("variant", C.c_int32), ("block_size", C.c_int32),
-                ("engine", C.c_int32), ("seg_buffer_mb", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("engine", C.c_int32), ("seg_buffer_mb", C.c_int32), ("chunk_records", C.c_int32),
+                ("reserved", C.c_int32 * 2)]
 
 
 class Rng(C.Structure):
@@ -113,7 +114,14 @@ SIGNATURES = {
     "st_timer_stop": (C.c_int, [_vp, C.POINTER(C.c_float)]),
     "st_kernel_timing": (C.c_int, [_vp, C.c_int]),
     "st_kernel_times": (C.c_int, [_vp, _vp, _vp]),
+    "st_set_checkpointing": (C.c_int, [_vp, C.c_int]),
+    "st_adjoint": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "st_block_sums": (C.c_int, [_vp, _vp, _i32, _vp]),
 }
+
+# parameter order of st_adjoint's gradient (15 per configuration)
+GRAD_PARAMS = ["Gx.x", "Gx.y", "Gx.z", "Gy.x", "Gy.y", "Gy.z", "Gz.x", "Gz.y", "Gz.z",
+               "E.x", "E.y", "E.z", "B0.x", "B0.y", "B0.z"]
 
 
 class DeviceBuffer:

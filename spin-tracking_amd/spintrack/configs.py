@@ -110,3 +110,20 @@ def dressing_cotracked(n: int = 262144, tf: float = 2.0, ioutInt: float = 0.1, s
     particles; the He-3 population continues the neutron population's seed
     range (seed + n)."""
     return [dressing(n, "n", tf, ioutInt, seed, tc), dressing(n, "he3", tf, ioutInt, seed + n, tc)]
+
+
+def adjoint_case(n: int = 131072, tf: float = 50.0, ioutInt: float = 0.5, diffuse: float = 0.01,
+                 seed: int = 4) -> Options:
+    """Differentiation workload: neutrons in the cell (5 m/s isotropic,
+    gravity, no gas) with diffuse = 0.01 walls, B0 = (3e-6,0,0) T,
+    E = (0,0,7.5e6) V/m, and the nine gradient coefficients dB_i/dx_j
+    uniform in [-1e-8, 1e-8] T/m (stream (seed, 0), rows Gx, Gy, Gz); 50 s
+    with an output every 0.5 s.  The spins start transverse to B0,
+    S = (0, 1, 0), so the precession phase atan2(Sz, Sy) of the ensemble is
+    defined."""
+    from .rng import uniforms
+
+    o = neutron_cell(n=n, tf=tf, ioutInt=ioutInt, seed=seed)
+    g = 1e-8 * (2.0 * uniforms(seed, 0, 9) - 1.0)
+    set_fields(o, yi=(0.0, 1.0, 0.0), diffuse=diffuse, Gx=tuple(g[0:3]), Gy=tuple(g[3:6]), Gz=tuple(g[6:9]))
+    return o

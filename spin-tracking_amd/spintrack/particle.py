@@ -55,18 +55,20 @@ class Particle:
     """GPU-resident ensemble of ``opt.numParticles`` particles."""
 
     def __init__(self, opt: Options, pulse_axis: int = 2, variant: int = L.VARIANT_EXACT,
-                 block_size: int = 0, engine: int = L.ENGINE_PIPELINE, seg_buffer_mb: int = 0):
+                 block_size: int = 0, engine: int = L.ENGINE_PIPELINE, seg_buffer_mb: int = 0,
+                 chunk_records: int = 0):
         self.lib = L.load()
         self.opt = Options.from_buffer_copy(bytes(opt))
         self.n = int(opt.numParticles)
-        self.ext = Ext(int(pulse_axis), int(variant), int(block_size), int(engine), int(seg_buffer_mb))
+        self.ext = Ext(int(pulse_axis), int(variant), int(block_size), int(engine), int(seg_buffer_mb),
+                       int(chunk_records))
         h = C.c_void_p()
         check(self.lib.st_create(C.byref(self.opt), C.byref(self.ext), C.byref(h)), None)
         self.h = h
 
     @classmethod
     def batch(cls, cfgs, pulse_axis: int = 2, variant: int = L.VARIANT_EXACT, block_size: int = 0,
-              seg_buffer_mb: int = 0) -> "Particle":
+              seg_buffer_mb: int = 0, chunk_records: int = 0) -> "Particle":
         """Batched handle (st_create_batch): the configurations ``cfgs`` (each
         an Options block with its own numParticles) back to back in one
         ensemble.  ``layout()`` gives the particle offsets."""
@@ -77,7 +79,8 @@ class Particle:
         self.opt = Options.from_buffer_copy(bytes(cfgs[0]))
         self.n = int(sum(int(c.numParticles) for c in cfgs))
         self.opt.numParticles = self.n
-        self.ext = Ext(int(pulse_axis), int(variant), int(block_size), L.ENGINE_PIPELINE, int(seg_buffer_mb))
+        self.ext = Ext(int(pulse_axis), int(variant), int(block_size), L.ENGINE_PIPELINE, int(seg_buffer_mb),
+                       int(chunk_records))
         h = C.c_void_p()
         check(self.lib.st_create_batch(arr, len(cfgs), C.byref(self.ext), C.byref(h)), None)
         self.h = h
@@ -190,6 +193,26 @@ class Particle:
     def copy_to_host(self, dev_ptr: int, nbytes: int) -> np.ndarray:
         out = np.empty(int(nbytes), np.uint8)
         check(self.lib.st_memcpy_d2h(self.h, _ptr(out), C.c_void_p(int(dev_ptr)), int(nbytes)), self.h)
+        return out
+
+    def set_checkpointing(self, enable: bool = True) -> None:
+        check(self.lib.st_set_checkpointing(self.h, 1 if enable else 0), self.h)
+
+    def adjoint(self, blocks_dev_ptr: int, seeds) -> np.ndarray:
+        """Gradient (n_cfg, 15) of a loss of the ensemble-mean spins, given
+        seeds[c, k, :] = dL/d<S>(t_k) of the last checkpointed run."""
+        n_cfg = len(self.layout()) - 1
+        s = np.ascontiguousarray(seeds, dtype=np.float64).reshape(n_cfg, -1, 3)
+        out = np.zeros((n_cfg, 15))
+        check(self.lib.st_adjoint(self.h, C.c_void_p(int(blocks_dev_ptr)), _ptr(s), _ptr(out)), self.h)
+        return out
+
+    def block_sums(self, blocks_dev_ptr: int, n_blocks: int) -> np.ndarray:
+        """(n_blocks, n_cfg, 4): valid count and sums of Sx, Sy, Sz per
+        snapshot block and configuration."""
+        n_cfg = len(self.layout()) - 1
+        out = np.zeros((int(n_blocks), n_cfg, 4))
+        check(self.lib.st_block_sums(self.h, C.c_void_p(int(blocks_dev_ptr)), int(n_blocks), _ptr(out)), self.h)
         return out
 
     def copy_device(self, dst_ptr: int, src_ptr: int, nbytes: int) -> None:
