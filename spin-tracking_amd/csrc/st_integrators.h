@@ -260,6 +260,24 @@ ST_HD __forceinline__ d3 krhs(const SimParams& P, const Phys& F, double t, doubl
     return cross3(y, w);
 }
 
+// Step-size arithmetic of the fast variant.  The error norm's scale factors,
+// the norm's square root and err^(1/8) only steer the controller, so they use
+// the hardware reciprocal / reciprocal square root refined by one Newton step
+// (about 1 ulp) instead of correctly rounded division and sqrt (about 10 and
+// 20 instructions each).
+__device__ __forceinline__ double fast_rcp(double d) {
+    const double r = __builtin_amdgcn_rcp(d);
+    return __builtin_fma(r, __builtin_fma(-d, r, 1.0), r);
+}
+__device__ __forceinline__ double fast_rsq(double x) {
+    const double r = __builtin_amdgcn_rsq(x);
+    // r * (1.5 - 0.5 * x * r^2)
+    const double hx = 0.5 * x;
+    return r * __builtin_fma(-hx * r, r, 1.5);
+}
+// x^(1/2) for x > 0
+__device__ __forceinline__ double fast_sqrt(double x) { return x * fast_rsq(x); }
+
 // Quadrature carried along an adjoint (backward) segment: moments
 // N_n = int u(s) s^n ds, n = 0, 1, 2, of the integrated vector over the
 // segment's own time s, by the step's 8th-order weights (b1, b6..b12) at the
@@ -373,9 +391,9 @@ ST_HD inline int dop_trial_fast_t(const SimParams& P, const Phys& F, DopState& D
     ST_V3(y1, y.x + bs.x, y.y + bs.y, y.z + bs.z);
     // error norm: with K = h*k the reference's |h| * err / sqrt(n*deno)
     // becomes errK / sqrt(n*denoK)
-    const double isx = 1.0 / (P.atol + P.rtol * max_d(fabs(y.x), fabs(y1.x)));
-    const double isy = 1.0 / (P.atol + P.rtol * max_d(fabs(y.y), fabs(y1.y)));
-    const double isz = 1.0 / (P.atol + P.rtol * max_d(fabs(y.z), fabs(y1.z)));
+    const double isx = fast_rcp(P.atol + P.rtol * __builtin_fmax(fabs(y.x), fabs(y1.x)));
+    const double isy = fast_rcp(P.atol + P.rtol * __builtin_fmax(fabs(y.y), fabs(y1.y)));
+    const double isz = fast_rcp(P.atol + P.rtol * __builtin_fmax(fabs(y.z), fabs(y1.z)));
     double ex = (bs.x - bhh1 * K1.x - bhh2 * K9.x - bhh3 * K12.x) * isx;
     double ey = (bs.y - bhh1 * K1.y - bhh2 * K9.y - bhh3 * K12.y) * isy;
     double ez = (bs.z - bhh1 * K1.z - bhh2 * K9.z - bhh3 * K12.z) * isz;
@@ -385,15 +403,15 @@ ST_HD inline int dop_trial_fast_t(const SimParams& P, const Phys& F, DopState& D
     ez = (er1 * K1.z + er6 * K6.z + er7 * K7.z + er8 * K8.z + er9 * K9.z + er10 * K10.z + er11 * K11.z + er12 * K12.z) * isz;
     const double e1 = ex * ex + ey * ey + ez * ez;
     const double deno = e1 + 0.01 * err2;
-    const double err = (deno <= 0.0) ? 0.0 : e1 / sqrt(3.0 * deno);
+    const double err = (deno <= 0.0) ? 0.0 : e1 * fast_rsq(3.0 * deno);
     double fac11;
     if (!P.pow_beta_on && P.expo1 == 0.125)
-        fac11 = sqrt(sqrt(sqrt(err)));
+        fac11 = (err > 0.0) ? fast_sqrt(fast_sqrt(fast_sqrt(err))) : 0.0;
     else
         fac11 = pow(err, P.expo1);
     double fac = P.pow_beta_on ? fac11 / pow(D.facold, P.beta) : fac11;
-    fac = max_d(P.facc2, min_d(P.facc1, fac / P.safe));
-    double hnew = h / fac;
+    fac = __builtin_fmax(P.facc2, __builtin_fmin(P.facc1, fac / P.safe));
+    double hnew = h * fast_rcp(fac);
     if (err <= 1.0 || P.fixed) {
         D.facold = max_d(err, 1.0E-4);
         nacc++;

@@ -1,7 +1,9 @@
 """Batched handles (st_create_batch) on the GPU: every configuration of a
-batch reproduces, bit for bit, the plain single-configuration run of the
-same options -- state, RNG, counters and every snapshot block -- so the
-batched form inherits the plain path's parity with the oracle."""
+batch reproduces the plain single-configuration run of the same options --
+state, RNG, counters and every snapshot block -- bit for bit in the exact
+variant, and with bit-exact trajectories, RNG and counters and spins within
+1e-12 in the fast variant, so the batched form inherits the plain path's
+parity with the oracle."""
 import numpy as np
 import pytest
 
@@ -44,11 +46,26 @@ def check_batch_equals_plain(cfgs, targets, stop_at, variant, pulse_axis=2):
         lo, hi = int(off[c]), int(off[c + 1])
         with st.Particle(cfg, variant=variant, pulse_axis=pulse_axis) as p:
             ps, pblocks, pcnt = run(p, targets, None if stop_at is None else stop_at[lo:hi])
-        assert same_bits(bs[lo:hi], ps), f"config {c}: state differs"
+        if variant == st.VARIANT_EXACT:
+            assert same_bits(bs[lo:hi], ps), f"config {c}: state differs"
+        else:
+            # the fast variant lets the compiler choose FMA contraction per
+            # kernel instantiation, so the batched and plain spin kernels may
+            # round an operation differently: the trajectory, RNG and counters
+            # stay bit-exact, spins agree to integration round-off
+            for name in ps.dtype.names:
+                if name != "S":
+                    assert same_bits(bs[lo:hi][name], ps[name]), f"config {c}: {name} differs"
+            dS = np.abs(st.xyz(bs[lo:hi]["S"]) - st.xyz(ps["S"]))
+            assert np.nanmax(dS, initial=0.0) < 1e-12, f"config {c}: max |dS| {np.nanmax(dS)}"
         for k in range(len(targets)):
             bb = block_slice(bblocks[k], lo, hi)
             for name in bb:
-                assert same_bits(bb[name], pblocks[k][name]), f"config {c} block {k} field {name}"
+                if variant == st.VARIANT_EXACT or name != "S":
+                    assert same_bits(bb[name], pblocks[k][name]), f"config {c} block {k} field {name}"
+                else:
+                    d = np.abs(st.xyz(bb[name]) - st.xyz(pblocks[k][name]))
+                    assert np.nanmax(d, initial=0.0) < 1e-12, f"config {c} block {k} S"
         for name in ("accepted", "rejected", "segments"):
             assert np.array_equal(bcnt[name][lo:hi], pcnt[name]), (c, name)
     return bs, off
