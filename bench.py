@@ -174,6 +174,81 @@ def bench_sweep(args, variant):
             "kernels": kt, "work": wk, "failures": fails}
 
 
+def bench_rk45(args, variant):
+    """The He-3 workload with integratorType 1 (the reference's RK45 with its
+    stale-k1 first stage) at the same tolerance, over the first output
+    interval(s): RK45 needs far more steps than DOP853 at rtol = 1e-12."""
+    import spintrack as st
+
+    opt = build_workload(args.rk45_particles, 1)
+    targets = st.output_times(opt)[: args.rk45_steps]
+    with st.Particle(opt, variant=variant) as w:   # warm-up
+        w.initParticles()
+        w.runSimulation(targets[0] * 0.01)
+        w.synchronize()
+    with st.Particle(opt, variant=variant) as p:
+        p.initParticles()
+        p.synchronize()
+        p.kernel_timing(True)
+        p.timer_start()
+        p.run(targets)
+        ms = p.timer_stop()
+        kt = p.kernel_times()
+        wk = p.work_counters()
+    psec = args.rk45_particles * (targets[-1] - opt.t0)
+    tf = FLOPS_RK45_TRIAL * (wk["accepted"] + wk["rejected"]) / (kt["spin_ms"] / 1e3) / 1e12
+    return {"workload": f"{args.rk45_particles} He-3 thermal, RK45 rtol=atol=1e-12, {len(targets)} x {opt.ioutInt} s",
+            "value": psec / (ms / 1e3), "ms": ms, "kernels": kt, "work": wk,
+            "spin_TFLOPs": tf, "spin_frac_fp64_peak": tf / FP64_PEAK_TFLOPS}
+
+
+def bench_dressing(args, variant):
+    """Spin dressing: 262,144 neutrons + 262,144 He-3 co-tracked, B0 =
+    (1e-6,0,0) T plus 1e-5 T * cos(2 pi 3000 t) along y, gas collisions,
+    2% diffuse walls, gravity, DOP853 rtol = atol = 1e-12, 2 s with an output
+    every 0.1 s.  Batched = both species in one handle and one st_run call;
+    plain = each species on its own handle, st_advance_to per interval."""
+    import numpy as np
+    import spintrack as st
+
+    cfgs = st.configs.dressing_cotracked(n=args.dressing_per)
+    targets = st.output_times(cfgs[0])
+    psec = float(sum(c.numParticles for c in cfgs)) * (targets[-1] - cfgs[0].t0)
+    b = st.Particle.batch(cfgs, variant=variant, pulse_axis=1)
+    b.initParticles()
+    bb = b.block_bytes()
+    blocks = st._lib.DeviceBuffer(bb * len(targets))
+    b.synchronize()
+    b.kernel_timing(True)
+    b.timer_start()
+    b.run(targets, blocks.ptr)
+    batched_ms = b.timer_stop()
+    kt = b.kernel_times()
+    wk = b.work_counters()
+    sums = b.block_sums(blocks.ptr, len(targets))
+    blocks.free()
+    b.close()
+    plain_ms = 0.0
+    for c in cfgs:
+        with st.Particle(c, variant=variant, pulse_axis=1) as p:
+            p.initParticles()
+            p.synchronize()
+            p.timer_start()
+            for t in targets:
+                p.runSimulation(t)
+            plain_ms += p.timer_stop()
+    pol = np.hypot(sums[-1, :, 2], sums[-1, :, 3]) / np.maximum(sums[-1, :, 0], 1)
+    tf = (FLOPS_ACC_GRAV * wk["accepted"] + FLOPS_REJ * wk["rejected"]) / (kt["spin_ms"] / 1e3) / 1e12
+    return {"workload": f"{len(cfgs)} x {args.dressing_per} (neutrons, He-3) co-tracked, B0 1e-6 T, "
+                        "1e-5 T cos(2 pi 3000 t) along y, gas, 2% diffuse, 2 s",
+            "particle_seconds": psec, "batched_value": psec / (batched_ms / 1e3), "batched_ms": batched_ms,
+            "plain_value": psec / (plain_ms / 1e3), "plain_ms": plain_ms,
+            "batched_over_plain": plain_ms / batched_ms, "kernels": kt, "work": wk,
+            "steps_per_particle_second": wk["accepted"] / psec,
+            "batched_spin_TFLOPs": tf, "batched_spin_frac_fp64_peak": tf / FP64_PEAK_TFLOPS,
+            "final_transverse_polarisation": {"neutron": float(pol[0]), "He-3": float(pol[1])}}
+
+
 ADJ_PARAMS = list(range(9)) + [11, 12]   # the nine gradient coefficients, E.z, B0.x
 
 
@@ -308,6 +383,12 @@ def main():
     ap.add_argument("--seg-buffer-mb", type=int, default=0, help="pipeline staging budget (0 = library default)")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched field-sweep leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiation leg")
+    ap.add_argument("--no-rk45", action="store_true", help="skip the RK45 leg")
+    ap.add_argument("--rk45-steps", type=int, default=1, help="output intervals of the RK45 leg")
+    ap.add_argument("--rk45-particles", type=int, default=131072,
+                    help="He-3 atoms of the RK45 leg (the first of the seeded ensemble)")
+    ap.add_argument("--no-dressing", action="store_true", help="skip the spin-dressing leg")
+    ap.add_argument("--dressing-per", type=int, default=262144, help="particles per species")
     ap.add_argument("--no-adjoint-fd", action="store_true", help="skip its finite-difference check")
     ap.add_argument("--adjoint-particles", type=int, default=131072)
     ap.add_argument("--adjoint-fd-ladder", action="store_true",
@@ -379,6 +460,16 @@ def main():
         sweep = bench_sweep(args, variant)
         print(f"[bench] sweep: {sweep}", file=sys.stderr, flush=True)
 
+    rk45 = None
+    if not args.no_rk45:
+        rk45 = bench_rk45(args, variant)
+        print(f"[bench] rk45: {rk45}", file=sys.stderr, flush=True)
+
+    dressing = None
+    if not args.no_dressing:
+        dressing = bench_dressing(args, variant)
+        print(f"[bench] dressing: {dressing}", file=sys.stderr, flush=True)
+
     adj = None
     if not args.no_adjoint:
         adj = bench_adjoint(args, variant)
@@ -412,6 +503,8 @@ def main():
         "cpu_baseline": cpu,
         "failures": res["failures"],
         "batched_sweep": sweep,
+        "rk45": rk45,
+        "dressing": dressing,
         "adjoint": adj,
     }
     # numpy scalars (counts, differences) print as plain JSON numbers

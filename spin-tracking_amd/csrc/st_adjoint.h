@@ -119,6 +119,10 @@ __global__ __launch_bounds__(256) void k_adj(const SimParams P, const DevState D
                    -(q.w0.z + Ts * (q.w1.z + Ts * q.w2.z))};
         qb.w1 = d3{q.w1.x + 2.0 * Ts * q.w2.x, q.w1.y + 2.0 * Ts * q.w2.y, q.w1.z + 2.0 * Ts * q.w2.z};
         qb.w2 = d3{-q.w2.x, -q.w2.y, -q.w2.z};
+        qb.pag = q.pag;
+        qb.pw = q.pw;
+        qb.paxis = q.paxis;
+        qb.pon = q.pon;
         const Seg sb{0.0, seg.p0, seg.v0};
         AdjQuad Q;
         Q.n0 = Q.n1 = Q.n2 = d3{0.0, 0.0, 0.0};
@@ -131,7 +135,7 @@ __global__ __launch_bounds__(256) void k_adj(const SimParams P, const DevState D
         dop.last = 0;
         dop.reject = 0;
         dop.nstep = 0;
-        dop.k1 = krhs_adj(F, rec.t, 0.0, u, qb, 1.0);
+        dop.k1 = poly_rhs(0.0, u, qb, qb.pon ? -(qb.pag * cos(qb.pw * rec.t)) : 0.0);
         int st = 0;
         while ((st = dop_trial_fast_t<true>(P, F, dop, u, h, sb, qb, nacc, nrej, &Q)) == 0) {
         }

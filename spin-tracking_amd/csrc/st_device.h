@@ -456,7 +456,36 @@ ST_HD __forceinline__ d3 rhs_exact(const SimParams& P, const Phys& F, double t, 
 // linear in them); the pulse, when on, is added separately.
 struct Poly {
     d3 w0, w1, w2;
+    // the pulse a*cos(w*t) along paxis, as gamma*a (pag) and w (pw); pon == 0:
+    // no pulse term at all
+    double pag, pw;
+    int32_t paxis, pon;
 };
+
+// sin and cos of a small angle, |th| <= 0.5, by their Taylor series (terms
+// to th^15 and th^14; the first omitted terms are below 3e-17 relative).
+ST_HD __forceinline__ void small_sincos(double th, double& s, double& c) {
+#pragma clang fp contract(fast)
+    const double t2 = th * th;
+    c = 1.0 + t2 * (-1.0 / 2 + t2 * (1.0 / 24 + t2 * (-1.0 / 720 + t2 * (1.0 / 40320 + t2 * (-1.0 / 3628800 +
+        t2 * (1.0 / 479001600 + t2 * (-1.0 / 87178291200.0)))))));
+    s = th * (1.0 + t2 * (-1.0 / 6 + t2 * (1.0 / 120 + t2 * (-1.0 / 5040 + t2 * (1.0 / 362880 + t2 * (-1.0 / 39916800 +
+        t2 * (1.0 / 6227020800.0 + t2 * (-1.0 / 1307674368000.0))))))));
+}
+
+// gamma*B along a segment plus the pulse value pv (already scaled like the
+// polynomial), crossed with y
+ST_HD __forceinline__ d3 poly_rhs(double tau, const d3& y, const Poly& q, double pv) {
+#pragma clang fp contract(fast)
+    d3 w{q.w0.x + tau * (q.w1.x + tau * q.w2.x), q.w0.y + tau * (q.w1.y + tau * q.w2.y),
+         q.w0.z + tau * (q.w1.z + tau * q.w2.z)};
+    if (q.pon) {
+        if (q.paxis == 0) w.x += pv;
+        else if (q.paxis == 1) w.y += pv;
+        else w.z += pv;
+    }
+    return cross3(y, w);
+}
 
 ST_HD inline Poly make_poly(const SimParams& P, const Phys& F, const Seg& S) {
 #pragma clang fp contract(fast)
@@ -480,22 +509,17 @@ ST_HD inline Poly make_poly(const SimParams& P, const Phys& F, const Seg& S) {
     q.w1 = d3{gm * (g1x + c1x * IC2), gm * g1y, gm * (g1z + c1z * IC2)};
     // 0.5 * G.a
     q.w2 = d3{gm * (0.5 * ay * F.G[1]), gm * (0.5 * ay * F.G[4]), gm * (0.5 * ay * F.G[7])};
+    q.pag = gm * F.pa;
+    q.pw = F.pw;
+    q.paxis = F.pulse_axis;
+    q.pon = F.pulse_on;
     return q;
 }
 
 ST_HD __forceinline__ d3 rhs_fast(const SimParams& P, const Phys& F, double t, const d3& y, double t0,
                                        const Poly& q) {
 #pragma clang fp contract(fast)
-    const double tau = t - t0;
-    d3 w{q.w0.x + tau * (q.w1.x + tau * q.w2.x), q.w0.y + tau * (q.w1.y + tau * q.w2.y),
-         q.w0.z + tau * (q.w1.z + tau * q.w2.z)};
-    if (F.pulse_on) {
-        const double pv = F.gamma * (F.pa * cos(F.pw * t));
-        if (F.pulse_axis == 0) w.x += pv;
-        else if (F.pulse_axis == 1) w.y += pv;
-        else w.z += pv;
-    }
-    return cross3(y, w);
+    return poly_rhs(t - t0, y, q, q.pon ? q.pag * cos(q.pw * t) : 0.0);
 }
 
 }  // namespace st

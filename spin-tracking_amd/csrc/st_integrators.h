@@ -246,19 +246,6 @@ ST_HD __forceinline__ void dop_begin_fast(const SimParams& P, const Phys& F, Dop
     D.k1 = rhs_fast(P, F, D.x, y, S.t0, q);
 }
 
-ST_HD __forceinline__ d3 krhs(const SimParams& P, const Phys& F, double t, double tau, const d3& y,
-                                   const Poly& hq, double h) {
-#pragma clang fp contract(fast)
-    d3 w{hq.w0.x + tau * (hq.w1.x + tau * hq.w2.x), hq.w0.y + tau * (hq.w1.y + tau * hq.w2.y),
-         hq.w0.z + tau * (hq.w1.z + tau * hq.w2.z)};
-    if (F.pulse_on) {
-        const double pv = h * (F.gamma * (F.pa * cos(F.pw * t)));
-        if (F.pulse_axis == 0) w.x += pv;
-        else if (F.pulse_axis == 1) w.y += pv;
-        else w.z += pv;
-    }
-    return cross3(y, w);
-}
 
 // Step-size arithmetic of the fast variant.  The error norm's scale factors,
 // the norm's square root and err^(1/8) only steer the controller, so they use
@@ -288,19 +275,6 @@ struct AdjQuad {
     double t_end;
 };
 
-ST_HD __forceinline__ d3 krhs_adj(const Phys& F, double t_phys, double tau, const d3& y, const Poly& hq,
-                                  double h) {
-#pragma clang fp contract(fast)
-    d3 w{hq.w0.x + tau * (hq.w1.x + tau * hq.w2.x), hq.w0.y + tau * (hq.w1.y + tau * hq.w2.y),
-         hq.w0.z + tau * (hq.w1.z + tau * hq.w2.z)};
-    if (F.pulse_on) {
-        const double pv = -(h * (F.gamma * (F.pa * cos(F.pw * t_phys))));
-        if (F.pulse_axis == 0) w.x += pv;
-        else if (F.pulse_axis == 1) w.y += pv;
-        else w.z += pv;
-    }
-    return cross3(y, w);
-}
 
 template <bool ADJ>
 ST_HD inline int dop_trial_fast_t(const SimParams& P, const Phys& F, DopState& D, d3& y, double& h,
@@ -317,11 +291,28 @@ ST_HD inline int dop_trial_fast_t(const SimParams& P, const Phys& F, DopState& D
     const double x = D.x;
     const double tx = x - S.t0;
     const Poly hq{d3{h * q.w0.x, h * q.w0.y, h * q.w0.z}, d3{h * q.w1.x, h * q.w1.y, h * q.w1.z},
-                  d3{h * q.w2.x, h * q.w2.y, h * q.w2.z}};
+                  d3{h * q.w2.x, h * q.w2.y, h * q.w2.z}, q.pag, q.pw, q.paxis, q.pon};
     const d3 K1{h * D.k1.x, h * D.k1.y, h * D.k1.z};
-#define KR(C, YY)                                                                        \
-    (ADJ ? krhs_adj(F, Q->t_end - (x + (C) * h), tx + (C) * h, (YY), hq, h)           \
-         : krhs(P, F, x + (C) * h, tx + (C) * h, (YY), hq, h))
+    // The pulse a*cos(w*t) at the stage times t = x + c*h (adjoint: at
+    // t_end - (x + c*h), with the field negated): one sincos per trial at the
+    // step start, then cos(phi0 +- w*c*h) by angle addition with the small
+    // stage offset's Taylor sincos; a direct cos when w*h is not small.
+    double pC = 1.0, pS = 0.0, hpag = 0.0;
+    bool psmall = true;
+    if (q.pon) {
+        hpag = (ADJ ? -h : h) * q.pag;
+        sincos(q.pw * (ADJ ? (Q->t_end - x) : x), &pS, &pC);
+        psmall = fabs(q.pw * h) <= 0.5;
+    }
+    auto pcos = [&](double c) -> double {
+        if (psmall) {
+            double sn, cs;
+            small_sincos(q.pw * (c * h), sn, cs);
+            return ADJ ? pC * cs + pS * sn : pC * cs - pS * sn;
+        }
+        return cos(q.pw * (ADJ ? Q->t_end - (x + c * h) : x + c * h));
+    };
+#define KR(C, YY) poly_rhs(tx + (C) * h, (YY), hq, q.pon ? hpag * pcos(C) : 0.0)
     // adjoint quadrature of this trial: sum_i b_i Y_i s_i^n
     d3 q0{0.0, 0.0, 0.0}, q1{0.0, 0.0, 0.0}, q2{0.0, 0.0, 0.0};
     auto quad = [&](double b, double si, const d3& Y) {
@@ -381,7 +372,8 @@ ST_HD inline int dop_trial_fast_t(const SimParams& P, const Phys& F, DopState& D
           y.y + (a121 * K1.y + a124 * K4.y + a125 * K5.y + a126 * K6.y + a127 * K7.y + a128 * K8.y + a129 * K9.y + a1210 * K10.y + a1211 * K11.y),
           y.z + (a121 * K1.z + a124 * K4.z + a125 * K5.z + a126 * K6.z + a127 * K7.z + a128 * K8.z + a129 * K9.z + a1210 * K10.z + a1211 * K11.z));
     quad(b12, tx + h, yy);
-    K12 = ADJ ? krhs_adj(F, Q->t_end - xph, tx + h, yy, hq, h) : krhs(P, F, xph, tx + h, yy, hq, h);
+    const double c12 = q.pon ? pcos(1.0) : 0.0;
+    K12 = poly_rhs(tx + h, yy, hq, hpag * c12);
 #undef KR
     d3 bs;
     ST_V3(bs, b1 * K1.x + b6 * K6.x + b7 * K7.x + b8 * K8.x + b9 * K9.x + b10 * K10.x + b11 * K11.x + b12 * K12.x,
@@ -418,12 +410,12 @@ ST_HD inline int dop_trial_fast_t(const SimParams& P, const Phys& F, DopState& D
         if constexpr (ADJ) {
             // derivative at the new point for the next trial (FSAL), and the
             // accepted step's quadrature
-            D.k1 = krhs_adj(F, Q->t_end - xph, tx + h, y1, q, 1.0);
+            D.k1 = poly_rhs(tx + h, y1, q, -(q.pag * c12));
             Q->n0 = d3{Q->n0.x + h * q0.x, Q->n0.y + h * q0.y, Q->n0.z + h * q0.z};
             Q->n1 = d3{Q->n1.x + h * q1.x, Q->n1.y + h * q1.y, Q->n1.z + h * q1.z};
             Q->n2 = d3{Q->n2.x + h * q2.x, Q->n2.y + h * q2.y, Q->n2.z + h * q2.z};
         } else {
-            D.k1 = rhs_fast(P, F, xph, y1, S.t0, q);
+            D.k1 = poly_rhs(tx + h, y1, q, q.pag * c12);
         }
         y = y1;
         D.x = xph;
