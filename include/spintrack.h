@@ -256,6 +256,22 @@ ST_API int st_adjoint(st_handle* h, const void* blocks_dev, const double* seeds,
  * Deterministic (fixed-order partial sums).  Waits for the result. */
 ST_API int st_block_sums(st_handle* h, const void* blocks_dev, int32_t n_blocks, double* out);
 
+/* Per-particle precession phase statistics over n_blocks consecutive
+ * snapshot blocks in device memory: the reference's analysis
+ * (benchmarks/CheckingFrequencyShiftFromEField.ipynb:5-27,
+ * scripts/nedmSim.py:151-178) as device reductions.  Particles with
+ * failure == 0 and a finite phase in the last block are selected; per block
+ * and configuration phi = atan2(Sz, Sy) is aligned to the left edge of the
+ * first fullest bin of a 1000-bin histogram (phases <= ref - pi gain 2 pi,
+ * >= ref + pi lose 2 pi), phases outside mean -+ 5 std are dropped, and
+ * out[(k*n_cfg + c)*6 + {0..5}] = mean (wrapped into [-pi, pi]), std,
+ * phases kept, phases selected, the reference angle, and the mean before
+ * wrapping.  phi_dev (optional, n_blocks*numParticles doubles of device
+ * memory) receives the selected phases (NaN where not selected).  Waits for
+ * the result. */
+#define ST_PHASE_STATS 6
+ST_API int st_phase_stats(st_handle* h, const void* blocks_dev, int32_t n_blocks, double* out, void* phi_dev);
+
 /* Waits for all work queued on the handle's stream. */
 ST_API int st_synchronize(st_handle* h);
 

@@ -913,6 +913,43 @@ ST_API int st_block_sums(st_handle* h, const void* blocks_dev, int32_t n_blocks,
     return ST_OK;
 }
 
+ST_API int st_phase_stats(st_handle* h, const void* blocks_dev, int32_t n_blocks, double* out, void* phi_dev) {
+    if (!h || !blocks_dev || !out || n_blocks <= 0) return fail(h, ST_E_ARG, "null argument or n_blocks <= 0");
+    std::vector<uint32_t> off = h->batch ? h->cfg_off : std::vector<uint32_t>{0, h->n};
+    double* phi = (double*)phi_dev;
+    uint32_t* d_off = nullptr;
+    double* d_out = nullptr;
+    bool own_phi = false;
+    hipError_t e = hipSuccess;
+    if (!phi) {
+        e = hipMalloc(&phi, sizeof(double) * (size_t)h->n * n_blocks);
+        own_phi = e == hipSuccess;
+    }
+    if (e == hipSuccess) e = hipMalloc(&d_off, sizeof(uint32_t) * off.size());
+    if (e == hipSuccess) e = hipMalloc(&d_out, sizeof(double) * PH_OUT * (size_t)n_blocks * h->n_cfg);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(d_off, off.data(), sizeof(uint32_t) * off.size(), hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_phase_prep, dim3((h->n + 255) / 256, n_blocks), dim3(256), 0, h->stream,
+                           (const char*)blocks_dev, (uint64_t)108 * h->n, h->n, n_blocks, phi);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_phase_stats, dim3(n_blocks, h->n_cfg), dim3(PH_THREADS), 0, h->stream, phi, h->n,
+                           d_off, d_out);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, d_out, sizeof(double) * PH_OUT * (size_t)n_blocks * h->n_cfg, hipMemcpyDeviceToHost,
+                           h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (own_phi) (void)hipFree(phi);
+    (void)hipFree(d_off);
+    (void)hipFree(d_out);
+    if (e != hipSuccess) return fail(h, ST_E_HIP, "phase statistics failed: %s", hipGetErrorString(e));
+    return ST_OK;
+}
+
 ST_API int st_synchronize(st_handle* h) {
     if (!h) return fail(nullptr, ST_E_ARG, "null handle");
     HIPCHK(h, hipStreamSynchronize(h->stream));

@@ -117,6 +117,7 @@ SIGNATURES = {
     "st_set_checkpointing": (C.c_int, [_vp, C.c_int]),
     "st_adjoint": (C.c_int, [_vp, _vp, _vp, _vp]),
     "st_block_sums": (C.c_int, [_vp, _vp, _i32, _vp]),
+    "st_phase_stats": (C.c_int, [_vp, _vp, _i32, _vp, _vp]),
 }
 
 # parameter order of st_adjoint's gradient (15 per configuration)

@@ -215,6 +215,17 @@ class Particle:
         check(self.lib.st_block_sums(self.h, C.c_void_p(int(blocks_dev_ptr)), int(n_blocks), _ptr(out)), self.h)
         return out
 
+    def phase_stats(self, blocks_dev_ptr: int, n_blocks: int, phi_dev_ptr: int = 0) -> np.ndarray:
+        """(n_blocks, n_cfg, 6): the reference's per-particle phase analysis
+        per snapshot block and configuration -- mean phase (wrapped), std,
+        phases kept after the 5-sigma cut, phases selected, reference angle
+        of the histogram alignment, mean before wrapping."""
+        n_cfg = len(self.layout()) - 1
+        out = np.zeros((int(n_blocks), n_cfg, 6))
+        check(self.lib.st_phase_stats(self.h, C.c_void_p(int(blocks_dev_ptr)), int(n_blocks), _ptr(out),
+                                      C.c_void_p(int(phi_dev_ptr)) if phi_dev_ptr else None), self.h)
+        return out
+
     def copy_device(self, dst_ptr: int, src_ptr: int, nbytes: int) -> None:
         check(self.lib.st_memcpy_d2d(self.h, C.c_void_p(int(dst_ptr)), C.c_void_p(int(src_ptr)),
                                      int(nbytes)), self.h)
