@@ -63,6 +63,13 @@ def run_gpu(opt, args, targets, persistent, variant, engine):
     import spintrack as st
 
     kw = dict(variant=variant, engine=engine, seg_buffer_mb=args.seg_buffer_mb)
+    if engine == st._lib.ENGINE_PIPELINE and not args.no_autotune:
+        # launch shape from the autotuner (probe: the first output interval)
+        from spintrack import autotune
+        shape = autotune.tune(opt, variant=variant, probe_intervals=2)
+        kw.update(chunk_records=shape["chunk_records"], block_size=shape["block_size"])
+        args.tuned_shape = {"chunk_records": shape["chunk_records"], "kin_block": shape["block_size"],
+                            "probe_ms": {f"{c}x{b}": round(v, 2) for (c, b), v in shape["timings"].items()}}
     # warm-up on its own ensemble (first-launch costs, clocks)
     if args.warmup > 0:
         wopt = type(opt).from_buffer_copy(bytes(opt))
@@ -137,7 +144,12 @@ def bench_sweep(args, variant):
         w.runSimulation(0.5)
         w.synchronize()
 
-    b = st.Particle.batch(cfgs, variant=variant)
+    # launch shape (chunk depth, kinematics lanes per block) from the autotuner,
+    # probing the first output interval of this workload
+    from spintrack import autotune
+    shape = autotune.tune(cfgs, variant=variant, stop_at=stop_at)
+    b = st.Particle.batch(cfgs, variant=variant, chunk_records=shape["chunk_records"],
+                          block_size=shape["block_size"])
     b.initParticles()
     b.set_stop_times(stop_at)
     b.synchronize()
@@ -171,6 +183,8 @@ def bench_sweep(args, variant):
             "plain_value": plain, "plain_ms": plain_ms, "batched_over_plain": batched / plain,
             "batched_spin_TFLOPs": spin_tf, "batched_spin_frac_fp64_peak": spin_tf / FP64_PEAK_TFLOPS,
             "batched_whole_run_TFLOPs": (spin_flops + FLOPS_EVENT * wk["segments"]) / (batched_ms / 1e3) / 1e12,
+            "launch_shape": {"chunk_records": shape["chunk_records"], "kin_block": shape["block_size"],
+                             "probe_ms": {f"{c}x{b_}": round(v, 3) for (c, b_), v in shape["timings"].items()}},
             "kernels": kt, "work": wk, "failures": fails}
 
 
@@ -382,6 +396,7 @@ def main():
     ap.add_argument("--engine", choices=["pipeline", "fused"], default="pipeline")
     ap.add_argument("--seg-buffer-mb", type=int, default=0, help="pipeline staging budget (0 = library default)")
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched field-sweep leg")
+    ap.add_argument("--no-autotune", action="store_true", help="library default launch shape for the main leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiation leg")
     ap.add_argument("--no-rk45", action="store_true", help="skip the RK45 leg")
     ap.add_argument("--rk45-steps", type=int, default=1, help="output intervals of the RK45 leg")
@@ -494,7 +509,8 @@ def main():
                                "DOP853 rtol=atol=1e-12, output every 0.5 s",
                    "particles": args.particles, "sim_seconds": targets[-1] - opt.t0,
                    "variant": args.variant, "integrator": "DOP853" if opt.integratorType == 0 else "RK45",
-                   "path": f"st_run, {args.engine} engine"},
+                   "path": f"st_run, {args.engine} engine",
+                   "launch_shape": getattr(args, "tuned_shape", None)},
         "plain_value": plain,
         "roofline": dict({"bound": "mfma", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS,
                           "unit": "TFLOP/s", "frac": achieved_tf / FP64_PEAK_TFLOPS, "traffic": traffic,

@@ -70,7 +70,8 @@ typedef struct st_ext {
     int32_t pulse_axis;     /* axis of pulse(t) = a*cos(w*t): 0,1,2; reference = 2 (z),
                                src/integrator.cpp:40-42 */
     int32_t variant;        /* ST_VARIANT_* */
-    int32_t block_size;     /* threads per workgroup of the fused plain kernel; 0 = 256 */
+    int32_t block_size;     /* threads per workgroup of the kinematics kernel (pipeline) or of the
+                               fused per-interval kernel; multiples of 64 up to 256; 0 = 256 */
     int32_t engine;         /* ST_ENGINE_* */
     int32_t seg_buffer_mb;  /* HBM budget of the pipeline's segment-record buffer; 0 = 16384 */
     int32_t chunk_records;  /* trajectory segments per particle per pipeline chunk (and between

@@ -411,6 +411,8 @@ static int run_pipeline(st_handle* h, RunArgs R) {
     if (integ != INT_NONE && (r = persistent_grid(h, spin_symbol(h->ext.variant, integ, h->batch), &h->grid_spin)))
         return r;
     const int bs = 256;
+    // lanes per workgroup of the kinematics kernel (a launch-shape knob)
+    const int kbs = h->ext.block_size > 0 ? std::min(256, (h->ext.block_size + 63) / 64 * 64) : 256;
     hipLaunchKernelGGL(k_begin_run, dim3((h->n + bs - 1) / bs), dim3(bs), 0, h->stream, h->D, h->n, h->d_list[0]);
     HIPCHK(h, hipGetLastError());
     PipeArgs A;
@@ -436,9 +438,9 @@ static int run_pipeline(st_handle* h, RunArgs R) {
         HIPCHK(h, hipMemsetAsync(h->d_small, 0, 4 * sizeof(uint32_t), h->stream));
         if (h->ktime_on) HIPCHK(h, hipEventRecord(h->kev[0], h->stream));
         if (h->batch)
-            hipLaunchKernelGGL(k_kin<true>, dim3((n_list + bs - 1) / bs), dim3(bs), 0, h->stream, h->P, h->D, A);
+            hipLaunchKernelGGL(k_kin<true>, dim3((n_list + kbs - 1) / kbs), dim3(kbs), 0, h->stream, h->P, h->D, A);
         else
-            hipLaunchKernelGGL(k_kin<false>, dim3((n_list + bs - 1) / bs), dim3(bs), 0, h->stream, h->P, h->D, A);
+            hipLaunchKernelGGL(k_kin<false>, dim3((n_list + kbs - 1) / kbs), dim3(kbs), 0, h->stream, h->P, h->D, A);
         HIPCHK(h, hipGetLastError());
         if (h->ktime_on) HIPCHK(h, hipEventRecord(h->kev[1], h->stream));
         if (h->ckpt_on && (r = save_ckpt(h, A.list, n_list))) return r;
