@@ -475,11 +475,12 @@ ST_HD __forceinline__ void small_sincos(double th, double& s, double& c) {
 
 // gamma*B along a segment plus the pulse value pv (already scaled like the
 // polynomial), crossed with y
+template <bool PULSE = true>
 ST_HD __forceinline__ d3 poly_rhs(double tau, const d3& y, const Poly& q, double pv) {
 #pragma clang fp contract(fast)
     d3 w{q.w0.x + tau * (q.w1.x + tau * q.w2.x), q.w0.y + tau * (q.w1.y + tau * q.w2.y),
          q.w0.z + tau * (q.w1.z + tau * q.w2.z)};
-    if (q.pon) {
+    if (PULSE && q.pon) {
         if (q.paxis == 0) w.x += pv;
         else if (q.paxis == 1) w.y += pv;
         else w.z += pv;

@@ -185,6 +185,7 @@ struct st_handle {
     uint64_t klaunch[2] = {0, 0};   // launches
     // batched handles: configurations back to back in particle order
     bool batch = false;
+    bool any_pulse = false;          // some configuration has a pulse term
     uint32_t n_cfg = 1;
     std::vector<uint32_t> cfg_off;   // n_cfg + 1 offsets
     Phys* d_phys = nullptr;
@@ -302,31 +303,41 @@ static const void* fused_symbol(int variant, int integ) {
                       : (const void*)&k_engine<VAR_EXACT, INT_RK45, true>;
 }
 
-template <int VAR, int INTEG, bool BATCH>
+// Spin kernels.  The fast DOP853 kernel has a form without the pulse term
+// (PULSE = false: fewer live registers) for handles whose configurations
+// have no pulse; the other kernels evaluate the pulse switch at run time.
+template <int VAR, int INTEG, bool BATCH, bool PULSE>
 static hipError_t launch_spin(st_handle* h, dim3 grid, const PipeArgs& A) {
-    hipLaunchKernelGGL((k_spin<VAR, INTEG, BATCH>), grid, dim3(256), 0, h->stream, h->P, h->D, A);
+    if constexpr (VAR == VAR_FAST && INTEG == INT_DOP853 && !PULSE)
+        hipLaunchKernelGGL((k_spin_o3<VAR, INTEG, BATCH, PULSE>), grid, dim3(256), 0, h->stream, h->P, h->D, A);
+    else
+        hipLaunchKernelGGL((k_spin<VAR, INTEG, BATCH, PULSE>), grid, dim3(256), 0, h->stream, h->P, h->D, A);
     return hipGetLastError();
 }
 using SpinFn = hipError_t (*)(st_handle*, dim3, const PipeArgs&);
 template <bool BATCH>
-static SpinFn pick_spin_b(int variant, int integ) {
-    if (variant == ST_VARIANT_FAST)
-        return integ == 1 ? launch_spin<VAR_FAST, INT_RK45, BATCH> : launch_spin<VAR_FAST, INT_DOP853, BATCH>;
-    return integ == 1 ? launch_spin<VAR_EXACT, INT_RK45, BATCH> : launch_spin<VAR_EXACT, INT_DOP853, BATCH>;
+static SpinFn pick_spin_b(int variant, int integ, bool pulse) {
+    if (variant == ST_VARIANT_FAST) {
+        if (integ == 1) return launch_spin<VAR_FAST, INT_RK45, BATCH, true>;
+        return pulse ? launch_spin<VAR_FAST, INT_DOP853, BATCH, true> : launch_spin<VAR_FAST, INT_DOP853, BATCH, false>;
+    }
+    return integ == 1 ? launch_spin<VAR_EXACT, INT_RK45, BATCH, true> : launch_spin<VAR_EXACT, INT_DOP853, BATCH, true>;
 }
-static SpinFn pick_spin(int variant, int integ, bool batch) {
-    return batch ? pick_spin_b<true>(variant, integ) : pick_spin_b<false>(variant, integ);
+static SpinFn pick_spin(int variant, int integ, bool batch, bool pulse) {
+    return batch ? pick_spin_b<true>(variant, integ, pulse) : pick_spin_b<false>(variant, integ, pulse);
 }
 template <bool BATCH>
-static const void* spin_symbol_b(int variant, int integ) {
-    if (variant == ST_VARIANT_FAST)
-        return integ == 1 ? (const void*)&k_spin<VAR_FAST, INT_RK45, BATCH>
-                          : (const void*)&k_spin<VAR_FAST, INT_DOP853, BATCH>;
-    return integ == 1 ? (const void*)&k_spin<VAR_EXACT, INT_RK45, BATCH>
-                      : (const void*)&k_spin<VAR_EXACT, INT_DOP853, BATCH>;
+static const void* spin_symbol_b(int variant, int integ, bool pulse) {
+    if (variant == ST_VARIANT_FAST) {
+        if (integ == 1) return (const void*)&k_spin<VAR_FAST, INT_RK45, BATCH, true>;
+        return pulse ? (const void*)&k_spin<VAR_FAST, INT_DOP853, BATCH, true>
+                     : (const void*)&k_spin_o3<VAR_FAST, INT_DOP853, BATCH, false>;
+    }
+    return integ == 1 ? (const void*)&k_spin<VAR_EXACT, INT_RK45, BATCH, true>
+                      : (const void*)&k_spin<VAR_EXACT, INT_DOP853, BATCH, true>;
 }
-static const void* spin_symbol(int variant, int integ, bool batch) {
-    return batch ? spin_symbol_b<true>(variant, integ) : spin_symbol_b<false>(variant, integ);
+static const void* spin_symbol(int variant, int integ, bool batch, bool pulse) {
+    return batch ? spin_symbol_b<true>(variant, integ, pulse) : spin_symbol_b<false>(variant, integ, pulse);
 }
 
 static int persistent_grid(st_handle* h, const void* sym, int* cache) {
@@ -408,7 +419,7 @@ static int run_pipeline(st_handle* h, RunArgs R) {
     }
     h->ckpt_run = R;
     const int integ = h->opt.integratorType;
-    if (integ != INT_NONE && (r = persistent_grid(h, spin_symbol(h->ext.variant, integ, h->batch), &h->grid_spin)))
+    if (integ != INT_NONE && (r = persistent_grid(h, spin_symbol(h->ext.variant, integ, h->batch, h->any_pulse), &h->grid_spin)))
         return r;
     const int bs = 256;
     // lanes per workgroup of the kinematics kernel (a launch-shape knob)
@@ -430,7 +441,7 @@ static int run_pipeline(st_handle* h, RunArgs R) {
     A.phys = h->d_phys;
     uint32_t n_list = h->n;
     int cur = 0;
-    const SpinFn spin = integ == INT_NONE ? nullptr : pick_spin(h->ext.variant, integ, h->batch);
+    const SpinFn spin = integ == INT_NONE ? nullptr : pick_spin(h->ext.variant, integ, h->batch, h->any_pulse);
     while (n_list > 0) {
         A.list = h->d_list[cur];
         A.n_list = n_list;
@@ -536,6 +547,7 @@ static int create_common(const st_options& opt, const st_ext& x, uint32_t n, st_
     h->ext = x;
     h->n = n;
     h->P = derive_params(opt, x);
+    h->any_pulse = h->P.ph.pulse_on != 0;
     h->state_bytes = (size_t)field_stride(h->n) * F_COUNT;
     hipError_t e = hipMalloc(&h->state, h->state_bytes);
     if (e == hipSuccess) e = hipMalloc(&h->d_work, N_WORK * sizeof(unsigned long long));
@@ -612,6 +624,8 @@ ST_API int st_create_batch(const st_options* cfgs, int32_t n_cfg, const st_ext* 
     st_handle* h = nullptr;
     if ((r = create_common(cfgs[0], x, (uint32_t)total, &h))) return r;
     h->batch = true;
+    h->any_pulse = false;
+    for (const Phys& f : phys) h->any_pulse = h->any_pulse || f.pulse_on != 0;
     h->n_cfg = (uint32_t)n_cfg;
     h->cfg_off = off;
     hipError_t e = hipMalloc(&h->d_phys, sizeof(Phys) * n_cfg);

@@ -239,7 +239,8 @@ ST_HD __forceinline__ void dop_begin_fast(const SimParams& P, const Phys& F, Dop
     D.x = S.t0;
     D.xf = t_end;
     D.facold = 1.0E-4;
-    D.posneg = sign_d(1.0, t_end - S.t0);
+    // sign(1, t_end - t0): a segment never runs backwards (t_end >= t0)
+    D.posneg = 1.0;
     D.last = 0;
     D.reject = 0;
     D.nstep = 0;
@@ -276,7 +277,8 @@ struct AdjQuad {
 };
 
 
-template <bool ADJ>
+// PULSE = false compiles the pulse term out (configurations without one).
+template <bool ADJ, bool PULSE = true>
 ST_HD inline int dop_trial_fast_t(const SimParams& P, const Phys& F, DopState& D, d3& y, double& h,
                                   const Seg& S, const Poly& q, uint32_t& nacc, uint32_t& nrej, AdjQuad* Q) {
 #pragma clang fp contract(fast)
@@ -299,7 +301,8 @@ ST_HD inline int dop_trial_fast_t(const SimParams& P, const Phys& F, DopState& D
     // stage offset's Taylor sincos; a direct cos when w*h is not small.
     double pC = 1.0, pS = 0.0, hpag = 0.0;
     bool psmall = true;
-    if (q.pon) {
+    const bool pon = PULSE && q.pon;
+    if (pon) {
         hpag = (ADJ ? -h : h) * q.pag;
         sincos(q.pw * (ADJ ? (Q->t_end - x) : x), &pS, &pC);
         psmall = fabs(q.pw * h) <= 0.5;
@@ -312,7 +315,7 @@ ST_HD inline int dop_trial_fast_t(const SimParams& P, const Phys& F, DopState& D
         }
         return cos(q.pw * (ADJ ? Q->t_end - (x + c * h) : x + c * h));
     };
-#define KR(C, YY) poly_rhs(tx + (C) * h, (YY), hq, q.pon ? hpag * pcos(C) : 0.0)
+#define KR(C, YY) poly_rhs<PULSE>(tx + (C) * h, (YY), hq, pon ? hpag * pcos(C) : 0.0)
     // adjoint quadrature of this trial: sum_i b_i Y_i s_i^n
     d3 q0{0.0, 0.0, 0.0}, q1{0.0, 0.0, 0.0}, q2{0.0, 0.0, 0.0};
     auto quad = [&](double b, double si, const d3& Y) {
@@ -372,8 +375,8 @@ ST_HD inline int dop_trial_fast_t(const SimParams& P, const Phys& F, DopState& D
           y.y + (a121 * K1.y + a124 * K4.y + a125 * K5.y + a126 * K6.y + a127 * K7.y + a128 * K8.y + a129 * K9.y + a1210 * K10.y + a1211 * K11.y),
           y.z + (a121 * K1.z + a124 * K4.z + a125 * K5.z + a126 * K6.z + a127 * K7.z + a128 * K8.z + a129 * K9.z + a1210 * K10.z + a1211 * K11.z));
     quad(b12, tx + h, yy);
-    const double c12 = q.pon ? pcos(1.0) : 0.0;
-    K12 = poly_rhs(tx + h, yy, hq, hpag * c12);
+    const double c12 = pon ? pcos(1.0) : 0.0;
+    K12 = poly_rhs<PULSE>(tx + h, yy, hq, hpag * c12);
 #undef KR
     d3 bs;
     ST_V3(bs, b1 * K1.x + b6 * K6.x + b7 * K7.x + b8 * K8.x + b9 * K9.x + b10 * K10.x + b11 * K11.x + b12 * K12.x,
@@ -410,12 +413,12 @@ ST_HD inline int dop_trial_fast_t(const SimParams& P, const Phys& F, DopState& D
         if constexpr (ADJ) {
             // derivative at the new point for the next trial (FSAL), and the
             // accepted step's quadrature
-            D.k1 = poly_rhs(tx + h, y1, q, -(q.pag * c12));
+            D.k1 = poly_rhs<PULSE>(tx + h, y1, q, -(q.pag * c12));
             Q->n0 = d3{Q->n0.x + h * q0.x, Q->n0.y + h * q0.y, Q->n0.z + h * q0.z};
             Q->n1 = d3{Q->n1.x + h * q1.x, Q->n1.y + h * q1.y, Q->n1.z + h * q1.z};
             Q->n2 = d3{Q->n2.x + h * q2.x, Q->n2.y + h * q2.y, Q->n2.z + h * q2.z};
         } else {
-            D.k1 = poly_rhs(tx + h, y1, q, q.pag * c12);
+            D.k1 = poly_rhs<PULSE>(tx + h, y1, q, q.pag * c12);
         }
         y = y1;
         D.x = xph;
@@ -433,9 +436,10 @@ ST_HD inline int dop_trial_fast_t(const SimParams& P, const Phys& F, DopState& D
     return 0;
 }
 
+template <bool PULSE = true>
 ST_HD __forceinline__ int dop_trial_fast(const SimParams& P, const Phys& F, DopState& D, d3& y, double& h,
                                          const Seg& S, const Poly& q, uint32_t& nacc, uint32_t& nrej) {
-    return dop_trial_fast_t<false>(P, F, D, y, h, S, q, nacc, nrej, nullptr);
+    return dop_trial_fast_t<false, PULSE>(P, F, D, y, h, S, q, nacc, nrej, nullptr);
 }
 
 // ---------------------------------------------------------------------------

@@ -322,8 +322,8 @@ __device__ __forceinline__ void spin_block(const RunArgs& R, const SpinLane& L) 
     if (R.blocks) write_block_spin(R.blocks + (uint64_t)L.k * R.block_bytes, R.n, L.pid, L.S);
 }
 
-template <int VAR, int INTEG, bool BATCH>
-__global__ __launch_bounds__(256) void k_spin(const SimParams P, const DevState D, const PipeArgs A) {
+template <int VAR, int INTEG, bool BATCH, bool PULSE>
+__device__ __forceinline__ void spin_body(const SimParams& P, const DevState& D, const PipeArgs& A) {
     const RunArgs& R = A.R;
     SpinLane L;
     // physics of the lane's current particle
@@ -335,10 +335,16 @@ __global__ __launch_bounds__(256) void k_spin(const SimParams P, const DevState 
     };
     DopState dop;
     RkState rk;
+    // end time of the segment being integrated (the record's time)
+    auto seg_end = [&]() -> double {
+        if constexpr (INTEG == INT_DOP853)
+            return dop.xf;
+        else
+            return rk.tf;
+    };
     Seg seg;
     Poly poly;
     uint64_t cur_flags = 0;
-    double cur_t = 0.0;
     unsigned long long cnt[4] = {0, 0, 0, 0};   // accepted, rejected, segments, trips
     enum { S_DONE = 0, S_NEXT = 1, S_INT = 2 };
     int phase = S_DONE;
@@ -368,7 +374,6 @@ __global__ __launch_bounds__(256) void k_spin(const SimParams P, const DevState 
                 }
                 const SegRec rec = get_rec(A.recs, A.C, L.pid, L.j);
                 cur_flags = rec.flags;
-                cur_t = rec.t;
                 if (rec.flags & R_FIRST) L.tempH = L.h;
                 if ((rec.flags & R_INTEGRATE) && !L.stopped) {
                     seg.t0 = L.t_spin;
@@ -405,20 +410,20 @@ __global__ __launch_bounds__(256) void k_spin(const SimParams P, const DevState 
             int st;
             if (INTEG == INT_DOP853) {
                 if (VAR == VAR_EXACT) st = dop_trial_exact(P, phys(), dop, L.S, L.tempH, seg, L.acc, L.rej);
-                else st = dop_trial_fast(P, phys(), dop, L.S, L.tempH, seg, poly, L.acc, L.rej);
+                else st = dop_trial_fast<PULSE>(P, phys(), dop, L.S, L.tempH, seg, poly, L.acc, L.rej);
             } else {
                 st = rk_trial<VAR>(P, phys(), rk, L.S, L.tempH, seg, poly, L.acc, L.rej);
             }
             if (st != 0) {
                 if (R.trace && L.pid == R.trace_pid)
-                    ST_TRACE(R, 2, L.t_spin, cur_t, d3{0, 0, 0}, d3{0, 0, 0}, L.S, L.tempH, L.acc, L.rej,
+                    ST_TRACE(R, 2, L.t_spin, seg_end(), d3{0, 0, 0}, d3{0, 0, 0}, L.S, L.tempH, L.acc, L.rej,
                              (char)(cur_flags >> 8), (char)(cur_flags >> 16), st, 0, 0);
                 // src/particle.cpp:593-612
                 if (P.keep) L.h = L.tempH;
                 struct {
                     uint64_t flags;
                 } rec{cur_flags};
-                L.t_spin = cur_t;
+                L.t_spin = seg_end();
                 if (st < 0) {
                     // the reference stops the particle here; the kinematics ran
                     // ahead, so k_fixup replays it up to this record
@@ -445,6 +450,18 @@ __global__ __launch_bounds__(256) void k_spin(const SimParams P, const DevState 
         }
     }
     flush_counters(R.work, cnt);
+}
+
+template <int VAR, int INTEG, bool BATCH, bool PULSE>
+__global__ __launch_bounds__(256) void k_spin(const SimParams P, const DevState D, const PipeArgs A) {
+    spin_body<VAR, INTEG, BATCH, PULSE>(P, D, A);
+}
+// The same kernel held to 168 VGPRs (three waves per SIMD); the fast DOP853
+// trial without the pulse term fits there, and the third wave hides the FP64
+// dependency latency the second one leaves exposed.
+template <int VAR, int INTEG, bool BATCH, bool PULSE>
+__global__ __launch_bounds__(256, 3) void k_spin_o3(const SimParams P, const DevState D, const PipeArgs A) {
+    spin_body<VAR, INTEG, BATCH, PULSE>(P, D, A);
 }
 
 // Rare path: a spin-integration failure at record j_fail of this chunk.
