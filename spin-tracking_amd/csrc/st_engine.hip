@@ -578,7 +578,18 @@ ST_API int st_create(const st_options* opt, const st_ext* ext, st_handle** out) 
     if (r) return r;
     st_ext x;
     if ((r = check_ext(ext, x))) return r;
-    return create_common(*opt, x, (uint32_t)opt->numParticles, out);
+    if ((r = create_common(*opt, x, (uint32_t)opt->numParticles, out))) return r;
+    // the spin kernel reads the physics block from device memory
+    st_handle* h = *out;
+    hipError_t e = hipMalloc(&h->d_phys, sizeof(Phys));
+    if (e == hipSuccess) e = hipMemcpy(h->d_phys, &h->P.ph, sizeof(Phys), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        fail(nullptr, ST_E_HIP, "device set-up failed: %s", hipGetErrorString(e));
+        st_destroy(h);
+        *out = nullptr;
+        return ST_E_HIP;
+    }
+    return ST_OK;
 }
 
 // The options that every configuration of a batch must share: geometry,

@@ -57,15 +57,16 @@ struct PipeArgs {
     const Phys* phys;        // batched handles: physics per configuration (F_CFG indexes it)
 };
 
-// The physics of particle pid: the launch's own block for a plain handle,
-// its configuration's entry for a batched one.
+// The physics of particle pid: its configuration's entry for a batched
+// handle, the single entry for a plain one (read through the table either
+// way: scalar loads where it is used instead of SGPRs held across loops).
 template <bool BATCH>
 __device__ __forceinline__ const Phys& lane_phys(const SimParams& P, const DevState& D, const Phys* table,
                                                  uint32_t pid) {
     if constexpr (BATCH)
         return table[(uint32_t)D.ld<uint64_t>(F_CFG, pid * 8u)];
     else
-        return P.ph;
+        return table[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -327,11 +328,14 @@ __device__ __forceinline__ void spin_body(const SimParams& P, const DevState& D,
     const RunArgs& R = A.R;
     SpinLane L;
     // physics of the lane's current particle
+    // (a plain handle's single entry is read through the same table: uniform
+    // scalar loads at each segment set-up instead of ~50 SGPRs held across
+    // the loop)
     auto phys = [&]() -> const Phys& {
         if constexpr (BATCH)
             return A.phys[L.cfg];
         else
-            return P.ph;
+            return A.phys[0];
     };
     DopState dop;
     RkState rk;
@@ -344,7 +348,7 @@ __device__ __forceinline__ void spin_body(const SimParams& P, const DevState& D,
     };
     Seg seg;
     Poly poly;
-    uint64_t cur_flags = 0;
+    uint32_t cur_flags = 0;   // low flag bits of the record being integrated (IEND, STOP, coll_type, wall_hit)
     unsigned long long cnt[4] = {0, 0, 0, 0};   // accepted, rejected, segments, trips
     enum { S_DONE = 0, S_NEXT = 1, S_INT = 2 };
     int phase = S_DONE;
@@ -373,7 +377,7 @@ __device__ __forceinline__ void spin_body(const SimParams& P, const DevState& D,
                     break;
                 }
                 const SegRec rec = get_rec(A.recs, A.C, L.pid, L.j);
-                cur_flags = rec.flags;
+                cur_flags = (uint32_t)rec.flags;
                 if (rec.flags & R_FIRST) L.tempH = L.h;
                 if ((rec.flags & R_INTEGRATE) && !L.stopped) {
                     seg.t0 = L.t_spin;
@@ -421,7 +425,7 @@ __device__ __forceinline__ void spin_body(const SimParams& P, const DevState& D,
                 // src/particle.cpp:593-612
                 if (P.keep) L.h = L.tempH;
                 struct {
-                    uint64_t flags;
+                    uint32_t flags;
                 } rec{cur_flags};
                 L.t_spin = seg_end();
                 if (st < 0) {
