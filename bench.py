@@ -36,6 +36,14 @@ FLOPS_ACC_NOGRAV = 1200.0
 FLOPS_REJ = 1340.0
 FLOPS_EVENT = 75.0
 FLOPS_RK45_TRIAL = 650.0
+# per trial of integratorTypes 2-5, counted the same way from the reference
+# formulation (gamma*B along a segment with gravity: 67 ops; rotation
+# quaternion from a vector: 18; quaternion product: 28; Rodrigues rotation of
+# a vector: 42; one CFET exponent: 30): MagnusCFET = 5 field samples + 16
+# exponents + 16 rotations + controller; RK45/RKF45 quaternion = 6 samples +
+# 11 rotation quaternions + 9 products + 2 vector rotations + controller;
+# RKF45 = RK45's count.
+FLOPS_TRIAL = {1: 650.0, 2: 1520.0, 3: 1000.0, 4: 650.0, 5: 1000.0}
 FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 vector, dense
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per segment record moved by k_spin (reads + writes), from the
@@ -51,8 +59,8 @@ def build_workload(n, integrator):
 
 
 def algorithmic_flops(opt, w):
-    if opt.integratorType == 1:
-        return FLOPS_RK45_TRIAL * (w["accepted"] + w["rejected"]) + FLOPS_EVENT * w["segments"]
+    if opt.integratorType in FLOPS_TRIAL:
+        return FLOPS_TRIAL[opt.integratorType] * (w["accepted"] + w["rejected"]) + FLOPS_EVENT * w["segments"]
     fa = FLOPS_ACC_GRAV if opt.gravity else FLOPS_ACC_NOGRAV
     return fa * w["accepted"] + FLOPS_REJ * w["rejected"] + FLOPS_EVENT * w["segments"]
 
@@ -214,6 +222,55 @@ def bench_rk45(args, variant):
     return {"workload": f"{args.rk45_particles} He-3 thermal, RK45 rtol=atol=1e-12, {len(targets)} x {opt.ioutInt} s",
             "value": psec / (ms / 1e3), "ms": ms, "kernels": kt, "work": wk,
             "spin_TFLOPs": tf, "spin_frac_fp64_peak": tf / FP64_PEAK_TFLOPS}
+
+
+# integratorType -> (name, rtol, simulated seconds): tolerances each method
+# can meet above hmin = 1e-8 (RKF45's "error" is the step's whole increment,
+# RKF45Q's the sixth rotation factor)
+INTEGRATOR_LEG = {2: ("MagnusCFET", 1e-12, 20.0), 3: ("RK45 quaternion", 1e-12, 20.0),
+                  4: ("RKF45", 1e-4, 0.01), 5: ("RKF45 quaternion", 1e-4, 0.05)}
+
+
+def bench_integrators(args, variant):
+    """integratorTypes 2-5 on neutrons of the reference cell (5 m/s,
+    specular walls, gravity, the same fields) with spins transverse to B0,
+    one output interval of the simulated time listed per type; the CPU
+    oracle on a 2048-particle sample of the same ensemble beside it."""
+    import spintrack as st
+    from spintrack import configs
+
+    out = {}
+    n = args.integrator_particles
+    for integ, (name, rtol, tf) in INTEGRATOR_LEG.items():
+        opt = configs.neutron_cell(n=n, tf=tf, ioutInt=tf)
+        opt.integratorType = integ
+        opt.rtol = opt.atol = rtol
+        opt.yi.x, opt.yi.y, opt.yi.z = 0.0, 0.6, 0.8
+        targets = st.output_times(opt)
+        with st.Particle(opt, variant=variant) as w:   # warm-up
+            w.initParticles()
+            w.runSimulation(targets[0] * 0.01)
+            w.synchronize()
+        with st.Particle(opt, variant=variant) as p:
+            p.initParticles()
+            p.synchronize()
+            p.kernel_timing(True)
+            p.timer_start()
+            p.run(targets)
+            ms = p.timer_stop()
+            kt = p.kernel_times()
+            wk = p.work_counters()
+        psec = n * (targets[-1] - opt.t0)
+        trials = wk["accepted"] + wk["rejected"]
+        tfl = FLOPS_TRIAL[integ] * trials / (kt["spin_ms"] / 1e3) / 1e12
+        cpu = cpu_baseline(opt, n, 2048, 1)
+        out[name] = {"integratorType": integ, "rtol": rtol, "simulated_s": tf, "value": psec / (ms / 1e3),
+                     "ms": ms, "kernels": kt, "trials_per_particle_s": trials / psec,
+                     "spin_TFLOPs": tfl, "spin_frac_fp64_peak": tfl / FP64_PEAK_TFLOPS,
+                     "cpu_oracle": cpu["value"], "cpu_sample": cpu["sample"],
+                     "gpu_over_cpu": psec / (ms / 1e3) / cpu["value"]}
+        print(f"[bench] integrator {name}: {out[name]}", file=sys.stderr, flush=True)
+    return {"workload": f"{n} neutrons, spins transverse to B0, one interval per type", "types": out}
 
 
 def bench_dressing(args, variant):
@@ -455,6 +512,8 @@ def main():
     ap.add_argument("--rk45-steps", type=int, default=1, help="output intervals of the RK45 leg")
     ap.add_argument("--rk45-particles", type=int, default=131072,
                     help="He-3 atoms of the RK45 leg (the first of the seeded ensemble)")
+    ap.add_argument("--no-integrators", action="store_true", help="skip the integratorType 2-5 leg")
+    ap.add_argument("--integrator-particles", type=int, default=131072)
     ap.add_argument("--no-dressing", action="store_true", help="skip the spin-dressing leg")
     ap.add_argument("--dressing-per", type=int, default=262144, help="particles per species")
     ap.add_argument("--no-adjoint-fd", action="store_true", help="skip its finite-difference check")
@@ -533,6 +592,10 @@ def main():
         rk45 = bench_rk45(args, variant)
         print(f"[bench] rk45: {rk45}", file=sys.stderr, flush=True)
 
+    integ = None
+    if not args.no_integrators:
+        integ = bench_integrators(args, variant)
+
     dressing = None
     if not args.no_dressing:
         dressing = bench_dressing(args, variant)
@@ -573,6 +636,7 @@ def main():
         "failures": res["failures"],
         "batched_sweep": sweep,
         "rk45": rk45,
+        "integrators": integ,
         "dressing": dressing,
         "adjoint": adj,
     }

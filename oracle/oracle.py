@@ -3,7 +3,7 @@
 TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()``
 and by bench.py's cpu_baseline leg, always as the checker and never as the
 thing measured or shipped.  The oracle restates the reference CPU path
-(src/particle.cpp:24-837, src/integrator.cpp:20-337 of the reference
+(src/particle.cpp:24-837, src/integrator.cpp:20-723, src/quaternion.cpp:128-164 of the reference
 repository) in plain C; see st_oracle.h for the line-by-line map.
 
 The structured numpy dtypes below mirror the C structs byte for byte:
@@ -106,6 +106,10 @@ def build() -> None:
 
 _lib = None
 
+# integratorType -> oracle function
+INTEGRATORS = ("or_integrate_dop", "or_integrate_rk45", "or_integrate_magnus",
+               "or_integrate_rk45q", "or_integrate_rkf45", "or_integrate_rkf45q")
+
 
 def lib():
     global _lib
@@ -135,9 +139,11 @@ def lib():
         L.or_sanity_check.argtypes = [vp] * 9
         L.or_interpolate.restype = None
         L.or_interpolate.argtypes = [d, d, _Coords, _Coords, vp, vp, vp]
+        L.or_cross_term.restype = _Coords
+        L.or_cross_term.argtypes = [d, vp, vp, d, _Coords, _Coords]
         L.or_bloch.restype = _Coords
         L.or_bloch.argtypes = [d, _Coords, vp, vp, d, _Coords, _Coords]
-        for nm in ("or_integrate_dop", "or_integrate_rk45"):
+        for nm in INTEGRATORS:
             f = getattr(L, nm)
             f.restype = i32
             f.argtypes = [d, d, vp, _Coords, _Coords, vp, vp, vp, vp, vp]
@@ -254,15 +260,24 @@ def bloch(opt, t, y, t0, p_old, v_old, pulse_axis=2):
     return np.array([r.x, r.y, r.z])
 
 
+def cross_term(opt, t, t0, p_old, v_old, pulse_axis=2):
+    """findCrossTerm: gamma * B along the segment (src/integrator.cpp:63-70)."""
+    e = _ext(pulse_axis)
+    r = lib().or_cross_term(t, _ptr(opt), C.addressof(e), t0, _c3(p_old), _c3(v_old))
+    return np.array([r.x, r.y, r.z])
+
+
 def integrate(opt, t0, tf, y, p_old, v_old, h, integrator=0, pulse_axis=2):
-    """One reference integrator call (integrateDOP / integrateRK45).
+    """One reference integrator call of integratorType ``integrator`` (0
+    integrateDOP, 1 integrateRK45, 2 integrateMagnusCFET, 3
+    integrateRK45Quaternion, 4 integrateRKF45, 5 integrateRKF45Quaternion).
     Returns (status, y_out, h_out, n_accepted, n_rejected)."""
     yy = np.array(y, dtype=np.float64)
     hh = np.array([h], dtype=np.float64)
     na = np.zeros(1, dtype=np.uint64)
     nr = np.zeros(1, dtype=np.uint64)
     e = _ext(pulse_axis)
-    f = lib().or_integrate_dop if integrator == 0 else lib().or_integrate_rk45
+    f = getattr(lib(), INTEGRATORS[integrator])
     st = f(t0, tf, _ptr(yy), _c3(p_old), _c3(v_old), _ptr(opt), C.addressof(e), _ptr(hh),
            _ptr(na), _ptr(nr))
     return st, yy, hh[0], int(na[0]), int(nr[0])

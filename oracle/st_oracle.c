@@ -429,9 +429,9 @@ void or_interpolate(double t, double t0, or_coords p_old, or_coords v_old,
     }
 }
 
-/* src/integrator.cpp:40-42 (pulse), :44-46 (grad), :63-70 (findCrossTerm),
- * :83-88 (Bloch): f = y x (gamma * (pulse + B0 + (1/c^2) v x E + G.r)) */
-or_coords or_bloch(double t, or_coords y, const or_options *opt, const or_ext *ext,
+/* src/integrator.cpp:40-42 (pulse), :44-46 (grad), :63-70 (findCrossTerm):
+ * gamma * (pulse + B0 + (1/c^2) v x E + G.r) */
+or_coords or_cross_term(double t, const or_options *opt, const or_ext *ext,
         double t0, or_coords p_old, or_coords v_old) {
     or_coords p, v;
     or_interpolate(t, t0, p_old, v_old, &p, &v, opt);
@@ -453,7 +453,13 @@ or_coords or_bloch(double t, or_coords y, const or_options *opt, const or_ext *e
     w.x = B.x * opt->gamma;
     w.y = B.y * opt->gamma;
     w.z = B.z * opt->gamma;
-    return cross3(y, w);
+    return w;
+}
+
+/* src/integrator.cpp:83-88 (Bloch): f = y x findCrossTerm */
+or_coords or_bloch(double t, or_coords y, const or_options *opt, const or_ext *ext,
+        double t0, or_coords p_old, or_coords v_old) {
+    return cross3(y, or_cross_term(t, opt, ext, t0, p_old, v_old));
 }
 
 /* ------------------------------------------------------------------ */
@@ -786,6 +792,307 @@ int or_integrate_rk45(double t0, double tf, or_coords *yio, or_coords p_old,
     *hio = h;
     return ret;
 }
+
+/* ------------------------------------------------------------------ */
+/* The other adaptive integrators: integrateRKF45 (src/integrator.cpp: */
+/* 339-426), integrateRK45Quaternion (:428-522), integrateRKF45-      */
+/* Quaternion (:524-618) and integrateMagnusCFET (:620-723).          */
+/* ------------------------------------------------------------------ */
+
+/* The step-size bounds these four share (e.g. :355-385).  Unlike RK45's
+ * (:267-292) the end-of-interval test comes first and skips the hmax/hmin
+ * checks, and a fixed step size never sets `stop`: with fixedStepSize the
+ * call runs until nstep exceeds nmax and returns -2, as the reference does.
+ * Returns -1 (h below hmin) or 0. */
+static int bounds_v2(const or_options *opt, double t, double tf, double *h, double *lastH,
+        bool *stop) {
+    const double endOfSimulDt = tf - t;
+    *lastH = *h;
+    if (opt->fixedStepSize) {
+        *h = opt->h;
+    } else if (*h >= endOfSimulDt) {
+        *h = endOfSimulDt;
+        *stop = true;
+    } else {
+        *lastH = *h;
+        *stop = false;
+        if (*h > opt->hmax) *h = std_min(*h, opt->hmax);
+        else if (*h < opt->hmin) return -1;
+    }
+    return 0;
+}
+
+/* The controller and accept test shared by all of them (e.g. :300-326).
+ * Returns 1 when the call ends (an accepted step with stop set). */
+static int control(const or_options *opt, double error, double *prev_ratio, double *t,
+        double *h, double lastH, bool *stop, or_coords *y, or_coords ynew,
+        uint64_t *n_acc, uint64_t *n_rej) {
+    error = std_max(error, 1.0E-16);
+    const double tol = opt->rtol;
+    const double ratio = tol / error;
+    double q = pow(ratio, 0.7 / 7.0) * pow(*prev_ratio, -0.4 / 7.0);
+    q = std_min(q, 4.0);
+    if ((q > 0.81 && error < 2 * tol) || opt->fixedStepSize) {
+        if (n_acc) (*n_acc)++;
+        *y = ynew;
+        *t += *h;
+        if (*stop) {
+            *h = lastH;
+            return 1;
+        }
+    } else {
+        if (n_rej) (*n_rej)++;
+        if (*stop) *stop = false;
+    }
+    if (!opt->fixedStepSize) *h = *h * q;
+    *prev_ratio = ratio;
+    return 0;
+}
+
+/* integrateRKF45 (:339-426), Fehlberg tableau include/coeff.h:220-256.
+ * As in RK45 the first stage is evaluated at the previous trial's sixth-stage
+ * input (zero on the first trial), and the "error" is the length of the
+ * fourth-order increment h*sum(CR_i k_i) (coefficients CR are the
+ * fourth-order weights, :398), not a difference of two solutions. */
+int or_integrate_rkf45(double t0, double tf, or_coords *yio, or_coords p_old,
+        or_coords v_old, const or_options *opt, const or_ext *ext, double *hio,
+        uint64_t *n_acc, uint64_t *n_rej) {
+    const double A2 = 1.0 / 4.0, A3 = 3.0 / 8.0, A4 = 12.0 / 13.0, A5 = 1.0, A6 = 0.5;
+    const double B21 = 0.25;
+    const double B31 = 3.0 / 32.0, B32 = 9.0 / 32.0;
+    const double B41 = 1932.0 / 2197.0, B42 = -7200.0 / 2197.0, B43 = 7296.0 / 2197.0;
+    const double B51 = 439.0 / 216.0, B52 = -8.0, B53 = 3680.0 / 513.0, B54 = -845.0 / 4104.0;
+    const double B61 = -8.0 / 27.0, B62 = 2.0, B63 = -3544.0 / 2565.0, B64 = 1859.0 / 4104.0,
+                 B65 = -11.0 / 40.0;
+    const double C1 = 16.0 / 135.0, C2 = 0.0, C3 = 6656.0 / 12825.0, C4 = 28561.0 / 56430.0,
+                 C5 = -9.0 / 50.0, C6 = 2.0 / 55.0;
+    const double R1 = 25.0 / 216.0, R2 = 0.0, R3 = 1408.0 / 2565.0, R4 = 2197.0 / 4104.0,
+                 R5 = -0.2, R6 = 0.0;
+    or_coords y = *yio;
+    double h = *hio, t = t0, lastH = h, prev_ratio = 1.0;
+    bool stop = false;
+    unsigned int nstep = 0;
+    or_coords k1, k2, k3, k4, k5, k6, yy1 = {0.0, 0.0, 0.0}, TE2, ws;
+    int ret = 0;
+    for (;;) {
+        nstep++;
+        if (nstep > opt->nmax) { ret = -2; break; }
+        if (bounds_v2(opt, t, tf, &h, &lastH, &stop)) { ret = -1; break; }
+        k1 = BLOCH(t, yy1);
+        { const double c = h * B21;
+          yy1.x = y.x + c * k1.x; yy1.y = y.y + c * k1.y; yy1.z = y.z + c * k1.z; }
+        k2 = BLOCH(t + A2 * h, yy1);
+        { const double c1 = h * B31, c2 = h * B32;
+          yy1.x = y.x + c1 * k1.x + c2 * k2.x;
+          yy1.y = y.y + c1 * k1.y + c2 * k2.y;
+          yy1.z = y.z + c1 * k1.z + c2 * k2.z; }
+        k3 = BLOCH(t + A3 * h, yy1);
+        { const double c1 = h * B41, c2 = h * B42, c3 = h * B43;
+          yy1.x = y.x + c1 * k1.x + c2 * k2.x + c3 * k3.x;
+          yy1.y = y.y + c1 * k1.y + c2 * k2.y + c3 * k3.y;
+          yy1.z = y.z + c1 * k1.z + c2 * k2.z + c3 * k3.z; }
+        k4 = BLOCH(t + A4 * h, yy1);
+        { const double c1 = h * B51, c2 = h * B52, c3 = h * B53, c4 = h * B54;
+          yy1.x = y.x + c1 * k1.x + c2 * k2.x + c3 * k3.x + c4 * k4.x;
+          yy1.y = y.y + c1 * k1.y + c2 * k2.y + c3 * k3.y + c4 * k4.y;
+          yy1.z = y.z + c1 * k1.z + c2 * k2.z + c3 * k3.z + c4 * k4.z; }
+        k5 = BLOCH(t + A5 * h, yy1);
+        { const double c1 = h * B61, c2 = h * B62, c3 = h * B63, c4 = h * B64, c5 = h * B65;
+          yy1.x = y.x + c1 * k1.x + c2 * k2.x + c3 * k3.x + c4 * k4.x + c5 * k5.x;
+          yy1.y = y.y + c1 * k1.y + c2 * k2.y + c3 * k3.y + c4 * k4.y + c5 * k5.y;
+          yy1.z = y.z + c1 * k1.z + c2 * k2.z + c3 * k3.z + c4 * k4.z + c5 * k5.z; }
+        k6 = BLOCH(t + A6 * h, yy1);
+        ws.x = y.x + h * (k1.x * C1 + k2.x * C2 + k3.x * C3 + k4.x * C4 + k5.x * C5 + k6.x * C6);
+        ws.y = y.y + h * (k1.y * C1 + k2.y * C2 + k3.y * C3 + k4.y * C4 + k5.y * C5 + k6.y * C6);
+        ws.z = y.z + h * (k1.z * C1 + k2.z * C2 + k3.z * C3 + k4.z * C4 + k5.z * C5 + k6.z * C6);
+        TE2.x = h * (R1 * k1.x + R2 * k2.x + R3 * k3.x + R4 * k4.x + R5 * k5.x + R6 * k6.x);
+        TE2.y = h * (R1 * k1.y + R2 * k2.y + R3 * k3.y + R4 * k4.y + R5 * k5.y + R6 * k6.y);
+        TE2.z = h * (R1 * k1.z + R2 * k2.z + R3 * k3.z + R4 * k4.z + R5 * k5.z + R6 * k6.z);
+        if (control(opt, len3(TE2), &prev_ratio, &t, &h, lastH, &stop, &y, ws, n_acc, n_rej))
+            break;
+    }
+    *yio = y;
+    *hio = h;
+    return ret;
+}
+
+/* Quaternions, src/quaternion.cpp:128-164 */
+typedef struct { double w, x, y, z; } or_quat;
+
+static or_quat q_mult(or_quat a, or_quat b) {
+    or_quat o;
+    o.w = a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z;
+    o.x = a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y;
+    o.y = a.w * b.y + a.y * b.w + a.z * b.x - a.x * b.z;
+    o.z = a.w * b.z + a.z * b.w + a.x * b.y - a.y * b.x;
+    return o;
+}
+
+/* q v q* (:137-148) */
+static or_coords qv_mult(or_quat q, or_coords v) {
+    const or_quat p = {0.0, v.x, v.y, v.z};
+    const or_quat c = {q.w, -q.x, -q.y, -q.z};
+    const or_quat r = q_mult(q_mult(q, p), c);
+    const or_coords o = {r.x, r.y, r.z};
+    return o;
+}
+
+/* The rotation by |k| dt about -k/|k| (:150-156) */
+static or_quat rodriguez_quat(or_coords k, double dt) {
+    const double angle = len3(k);
+    const or_coords n = {k.x / angle, k.y / angle, k.z / angle};
+    const double hh = angle * dt;
+    const double s = -sin(hh / 2.0);
+    const or_quat o = {cos(hh / 2.0), n.x * s, n.y * s, n.z * s};
+    return o;
+}
+
+/* v rotated by |k| about -k/|k| (:158-163) */
+static or_coords rodriguez(or_coords k, or_coords v) {
+    const double angle = len3(k);
+    const double s = sin(angle);
+    const double c = cos(angle);
+    const or_coords vk = cross3(v, k);
+    const double sa = s / angle;
+    const double f = (dot3(k, v) * (1.0 - c)) / (angle * angle);
+    const or_coords o = {(v.x * c + vk.x * sa) + k.x * f, (v.y * c + vk.y * sa) + k.y * f,
+                         (v.z * c + vk.z * sa) + k.z * f};
+    return o;
+}
+
+/* integrateRK45Quaternion / integrateRKF45Quaternion: six field samples
+ * k_i = findCrossTerm(t + A_i h); the step is y rotated by the product of
+ * rotations R(k_i, h W_i), i = 1..6, applied k1 first; the comparison
+ * solution drops the sixth factor and uses weights E_i.  RK45 tableau:
+ * A from RK45COEF, W = CH, E = C; RKF45 tableau: A from RKF45COEF, W = E = C
+ * (include/coeff.h:178-256). */
+static int integrate_quat(double t0, double tf, or_coords *yio, or_coords p_old,
+        or_coords v_old, const or_options *opt, const or_ext *ext, double *hio,
+        uint64_t *n_acc, uint64_t *n_rej, const double *A, const double *W, const double *E) {
+    or_coords y = *yio;
+    double h = *hio, t = t0, lastH = h, prev_ratio = 1.0;
+    bool stop = false;
+    unsigned int nstep = 0;
+    int ret = 0;
+    for (;;) {
+        nstep++;
+        if (nstep > opt->nmax) { ret = -2; break; }
+        if (bounds_v2(opt, t, tf, &h, &lastH, &stop)) { ret = -1; break; }
+        or_coords k[6];
+        k[0] = or_cross_term(t, opt, ext, t0, p_old, v_old);
+        for (int i = 1; i < 6; i++) k[i] = or_cross_term(t + A[i] * h, opt, ext, t0, p_old, v_old);
+        or_quat qw = rodriguez_quat(k[0], h * W[0]);
+        or_quat qe = rodriguez_quat(k[0], h * E[0]);
+        for (int i = 1; i < 5; i++) {
+            qw = q_mult(rodriguez_quat(k[i], h * W[i]), qw);
+            qe = q_mult(rodriguez_quat(k[i], h * E[i]), qe);
+        }
+        qw = q_mult(rodriguez_quat(k[5], h * W[5]), qw);
+        const or_coords ws = qv_mult(qw, y);
+        const or_coords te = qv_mult(qe, y);
+        const or_coords d = {ws.x - te.x, ws.y - te.y, ws.z - te.z};
+        if (control(opt, len3(d), &prev_ratio, &t, &h, lastH, &stop, &y, ws, n_acc, n_rej))
+            break;
+    }
+    *yio = y;
+    *hio = h;
+    return ret;
+}
+
+int or_integrate_rk45q(double t0, double tf, or_coords *yio, or_coords p_old,
+        or_coords v_old, const or_options *opt, const or_ext *ext, double *hio,
+        uint64_t *n_acc, uint64_t *n_rej) {
+    static const double A[6] = {0.0, 2.0 / 9.0, 1.0 / 3.0, 3.0 / 4.0, 1.0, 5.0 / 6.0};
+    static const double W[6] = {47.0 / 450.0, 0.0, 12.0 / 25.0, 32.0 / 225.0, 1.0 / 30.0, 6.0 / 25.0};
+    static const double E[5] = {1.0 / 9.0, 0.0, 9.0 / 20.0, 16.0 / 45.0, 1.0 / 12.0};
+    return integrate_quat(t0, tf, yio, p_old, v_old, opt, ext, hio, n_acc, n_rej, A, W, E);
+}
+
+int or_integrate_rkf45q(double t0, double tf, or_coords *yio, or_coords p_old,
+        or_coords v_old, const or_options *opt, const or_ext *ext, double *hio,
+        uint64_t *n_acc, uint64_t *n_rej) {
+    static const double A[6] = {0.0, 1.0 / 4.0, 3.0 / 8.0, 12.0 / 13.0, 1.0, 0.5};
+    static const double W[6] = {16.0 / 135.0, 0.0, 6656.0 / 12825.0, 28561.0 / 56430.0,
+                                -9.0 / 50.0, 2.0 / 55.0};
+    return integrate_quat(t0, tf, yio, p_old, v_old, opt, ext, hio, n_acc, n_rej, A, W, W);
+}
+
+/* integrateMagnusCFET (:620-723): the commutator-free exponential
+ * integrator of order 8 (11 exponentials) with an embedded order-6 one (5
+ * exponentials) from Alvermann & Fehske, J. Comput. Phys. 230 (2011) 5930
+ * (arXiv:1102.5071), on the field sampled at the 5 Gauss-Legendre nodes;
+ * coefficients include/coeff.h:258-321 (quadrature weights folded in). */
+static const double GL5X[5] = {0.046910077030668074, 0.2307653449471585, 0.5,
+                               0.7692346550528415, 0.9530899229693319};
+static const double CF8[6][5] = {
+    {-0.005984812750244602, 0.020217897716672675, -0.03646660716686794, 0.04245408899634467,
+     0.14949496424802844},
+    {0.02512881778391826, -0.08955337580937157, 0.19032828600393248, 0.28920194841768715,
+     -0.03568486888016069},
+    {-0.04972613261435144, 0.6572496090804754, 0.11725149008224456, -0.43611911073577403,
+     0.18080345083145516},
+    {0.0464837870954061, -0.6265183532355708, -0.11144873822286155, 0.4154988625423615,
+     -0.17224148557040517},
+    {0.015567390500242426, -0.0677120111619075, -0.15241336773272682, -0.08803798509100481,
+     -0.0013284996209207986},
+    {-0.034049168494873015, 0.12263276452977083, 0.2699423185170027, 0.12263276452977094,
+     -0.03404916849487304}};
+static const double CF6[3][5] = {
+    {-0.007431519425246311, 0.022452824849980513, -0.03620393417193995, 0.0409779901800809,
+     0.1516046385671249},
+    {0.03095749686312404, -0.10116885619385878, 0.1997455955716367, 0.3010242365525433,
+     -0.055594729593982636},
+    {-0.001072443882925378, -0.023971860139062367, -0.04263887835494916, -0.023971860139062388,
+     -0.0010724438829253797}};
+
+/* (c_1 B1 + c_2 B2 + c_3 B3 + c_4 B4 + c_5 B5) * h, summed left to right;
+ * rev takes the row backwards (c_5 B1 + ... + c_1 B5) */
+static or_coords cfet_exp(const double *c, int rev, const or_coords *B, double h) {
+    double cc[5];
+    for (int i = 0; i < 5; i++) cc[i] = rev ? c[4 - i] : c[i];
+    or_coords o;
+    o.x = ((((cc[0] * B[0].x + cc[1] * B[1].x) + cc[2] * B[2].x) + cc[3] * B[3].x) + cc[4] * B[4].x) * h;
+    o.y = ((((cc[0] * B[0].y + cc[1] * B[1].y) + cc[2] * B[2].y) + cc[3] * B[3].y) + cc[4] * B[4].y) * h;
+    o.z = ((((cc[0] * B[0].z + cc[1] * B[1].z) + cc[2] * B[2].z) + cc[3] * B[3].z) + cc[4] * B[4].z) * h;
+    return o;
+}
+
+int or_integrate_magnus(double t0, double tf, or_coords *yio, or_coords p_old,
+        or_coords v_old, const or_options *opt, const or_ext *ext, double *hio,
+        uint64_t *n_acc, uint64_t *n_rej) {
+    or_coords y = *yio;
+    double h = *hio, t = t0, lastH = h, prev_ratio = 1.0;
+    bool stop = false;
+    unsigned int nstep = 0;
+    int ret = 0;
+    for (;;) {
+        nstep++;
+        if (nstep > opt->nmax) { ret = -2; break; }
+        if (bounds_v2(opt, t, tf, &h, &lastH, &stop)) { ret = -1; break; }
+        or_coords B[5];
+        for (int i = 0; i < 5; i++) B[i] = or_cross_term(t + GL5X[i] * h, opt, ext, t0, p_old, v_old);
+        /* order 8: k11 = row 1 reversed is applied first, then rows 2..5
+         * reversed (k10..k7), row 6 (k6), rows 5..1 forward (k5..k1) */
+        or_coords y8 = y;
+        for (int r = 0; r < 5; r++) y8 = rodriguez(cfet_exp(CF8[r], 1, B, h), y8);
+        y8 = rodriguez(cfet_exp(CF8[5], 0, B, h), y8);
+        for (int r = 4; r >= 0; r--) y8 = rodriguez(cfet_exp(CF8[r], 0, B, h), y8);
+        /* order 6: k5 = row 1 reversed, k4 = row 2 reversed, k3 = row 3, k2 =
+         * row 2, k1 = row 1 */
+        or_coords y6 = y;
+        y6 = rodriguez(cfet_exp(CF6[0], 1, B, h), y6);
+        y6 = rodriguez(cfet_exp(CF6[1], 1, B, h), y6);
+        y6 = rodriguez(cfet_exp(CF6[2], 0, B, h), y6);
+        y6 = rodriguez(cfet_exp(CF6[1], 0, B, h), y6);
+        y6 = rodriguez(cfet_exp(CF6[0], 0, B, h), y6);
+        const or_coords d = {y8.x - y6.x, y8.y - y6.y, y8.z - y6.z};
+        if (control(opt, len3(d), &prev_ratio, &t, &h, lastH, &stop, &y, y8, n_acc, n_rej))
+            break;
+    }
+    *yio = y;
+    *hio = h;
+    return ret;
+}
 #undef BLOCH
 
 /* ------------------------------------------------------------------ */
@@ -848,8 +1155,9 @@ void or_init_particles(const or_options *opt, const or_arrays *a, int n,
     }
 }
 
-/* src/particle.cpp:712-837 (integratorType 0 and 1; other types do nothing
- * to the spin here). */
+/* src/particle.cpp:712-837 (integratorTypes 0-5; type 6, the spectral path,
+ * is not restated, and types outside 0-6 leave the spin alone as the
+ * reference's final else does). */
 void or_run_simulation(const or_options *opt, const or_ext *ext, const or_arrays *a,
         int n, double nextTOut, const double *stop_at, double prevTOut,
         uint64_t *n_ode, int nthreads) {
@@ -893,6 +1201,18 @@ void or_run_simulation(const or_options *opt, const or_ext *ext, const or_arrays
                             &tempH, &nacc, &nrej);
                 else if (opt->integratorType == 1)
                     spinResult = or_integrate_rk45(t_old, t, &S, pos_old, v_old, opt, ext,
+                            &tempH, &nacc, &nrej);
+                else if (opt->integratorType == 2)
+                    spinResult = or_integrate_magnus(t_old, t, &S, pos_old, v_old, opt, ext,
+                            &tempH, &nacc, &nrej);
+                else if (opt->integratorType == 3)
+                    spinResult = or_integrate_rk45q(t_old, t, &S, pos_old, v_old, opt, ext,
+                            &tempH, &nacc, &nrej);
+                else if (opt->integratorType == 4)
+                    spinResult = or_integrate_rkf45(t_old, t, &S, pos_old, v_old, opt, ext,
+                            &tempH, &nacc, &nrej);
+                else if (opt->integratorType == 5)
+                    spinResult = or_integrate_rkf45q(t_old, t, &S, pos_old, v_old, opt, ext,
                             &tempH, &nacc, &nrej);
                 else
                     spinResult = 0;

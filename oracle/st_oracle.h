@@ -101,12 +101,32 @@ void or_sanity_check(double *t_old, double *t, or_coords *pos_old, or_coords *po
 /* ---- spin ODE ---- */
 void or_interpolate(double t, double t0, or_coords p_old, or_coords v_old,
         or_coords *p_out, or_coords *v_out, const or_options *opt);
+or_coords or_cross_term(double t, const or_options *opt, const or_ext *ext,
+        double t0, or_coords p_old, or_coords v_old);
 or_coords or_bloch(double t, or_coords y, const or_options *opt, const or_ext *ext,
         double t0, or_coords p_old, or_coords v_old);
+/* integratorType 0: integrateDOP, src/integrator.cpp:90-249 */
 int or_integrate_dop(double t0, double tf, or_coords *y, or_coords p_old,
         or_coords v_old, const or_options *opt, const or_ext *ext, double *h,
         uint64_t *n_acc, uint64_t *n_rej);
+/* integratorType 1: integrateRK45, :251-337 */
 int or_integrate_rk45(double t0, double tf, or_coords *y, or_coords p_old,
+        or_coords v_old, const or_options *opt, const or_ext *ext, double *h,
+        uint64_t *n_acc, uint64_t *n_rej);
+/* integratorType 2: integrateMagnusCFET, :620-723 */
+int or_integrate_magnus(double t0, double tf, or_coords *y, or_coords p_old,
+        or_coords v_old, const or_options *opt, const or_ext *ext, double *h,
+        uint64_t *n_acc, uint64_t *n_rej);
+/* integratorType 3: integrateRK45Quaternion, :428-522 */
+int or_integrate_rk45q(double t0, double tf, or_coords *y, or_coords p_old,
+        or_coords v_old, const or_options *opt, const or_ext *ext, double *h,
+        uint64_t *n_acc, uint64_t *n_rej);
+/* integratorType 4: integrateRKF45, :339-426 */
+int or_integrate_rkf45(double t0, double tf, or_coords *y, or_coords p_old,
+        or_coords v_old, const or_options *opt, const or_ext *ext, double *h,
+        uint64_t *n_acc, uint64_t *n_rej);
+/* integratorType 5: integrateRKF45Quaternion, :524-618 */
+int or_integrate_rkf45q(double t0, double tf, or_coords *y, or_coords p_old,
         or_coords v_old, const or_options *opt, const or_ext *ext, double *h,
         uint64_t *n_acc, uint64_t *n_rej);
 

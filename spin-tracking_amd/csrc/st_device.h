@@ -33,7 +33,7 @@ constexpr double DBL_MAX_D = 1.79769313486231570815e+308; // DBL_MAX
 // arithmetic variants of the lane engine
 enum { VAR_EXACT = 0, VAR_FAST = 1 };
 // integrator types (options.integratorType; src/particle.cpp:568-592)
-enum { INT_NONE = -1, INT_DOP853 = 0, INT_RK45 = 1 };
+enum { INT_NONE = -1, INT_DOP853 = 0, INT_RK45 = 1, INT_MAGNUS = 2, INT_RK45Q = 3, INT_RKF45 = 4, INT_RKF45Q = 5 };
 
 struct d3 {
     double x, y, z;
@@ -413,9 +413,9 @@ struct Seg {
     d3 p0, v0;     // position and velocity at t0 (pos_old, v_old)
 };
 
-// Exact: interpolate -> grad -> pulse + B0 + (1/c^2) v x E + G.r -> gamma*B ->
-// y x (gamma B), in the reference's order.
-ST_HD __forceinline__ d3 rhs_exact(const SimParams& P, const Phys& F, double t, const d3& y, const Seg& S) {
+// Exact: interpolate -> grad -> pulse + B0 + (1/c^2) v x E + G.r -> gamma*B
+// (findCrossTerm, src/integrator.cpp:63-70), in the reference's order.
+ST_HD __forceinline__ d3 omega_exact(const SimParams& P, const Phys& F, double t, const Seg& S) {
     const double d = t - S.t0;
     d3 p, v;
     if (P.gravity) {
@@ -447,8 +447,12 @@ ST_HD __forceinline__ d3 rhs_exact(const SimParams& P, const Phys& F, double t, 
     const double Bx = ((plx + F.B0x) + cx * IC2) + gx;
     const double By = ((ply + F.B0y) + cy * IC2) + gy;
     const double Bz = ((plz + F.B0z) + cz * IC2) + gz;
-    const d3 w{Bx * F.gamma, By * F.gamma, Bz * F.gamma};
-    return cross3(y, w);
+    return d3{Bx * F.gamma, By * F.gamma, Bz * F.gamma};
+}
+
+// Bloch (:83-88): y x (gamma B)
+ST_HD __forceinline__ d3 rhs_exact(const SimParams& P, const Phys& F, double t, const d3& y, const Seg& S) {
+    return cross3(y, omega_exact(P, F, t, S));
 }
 
 // Fast: along a segment gamma*B is a quadratic polynomial in tau = t - t0
@@ -521,6 +525,21 @@ ST_HD __forceinline__ d3 rhs_fast(const SimParams& P, const Phys& F, double t, c
                                        const Poly& q) {
 #pragma clang fp contract(fast)
     return poly_rhs(t - t0, y, q, q.pon ? q.pag * cos(q.pw * t) : 0.0);
+}
+
+// gamma*B itself from the polynomial (plus the pulse when on)
+ST_HD __forceinline__ d3 omega_fast(double t, double t0, const Poly& q) {
+#pragma clang fp contract(fast)
+    const double tau = t - t0;
+    d3 w{q.w0.x + tau * (q.w1.x + tau * q.w2.x), q.w0.y + tau * (q.w1.y + tau * q.w2.y),
+         q.w0.z + tau * (q.w1.z + tau * q.w2.z)};
+    if (q.pon) {
+        const double pv = q.pag * cos(q.pw * t);
+        if (q.paxis == 0) w.x += pv;
+        else if (q.paxis == 1) w.y += pv;
+        else w.z += pv;
+    }
+    return w;
 }
 
 }  // namespace st

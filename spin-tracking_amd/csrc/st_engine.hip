@@ -315,29 +315,46 @@ static hipError_t launch_spin(st_handle* h, dim3 grid, const PipeArgs& A) {
     return hipGetLastError();
 }
 using SpinFn = hipError_t (*)(st_handle*, dim3, const PipeArgs&);
-template <bool BATCH>
-static SpinFn pick_spin_b(int variant, int integ, bool pulse) {
-    if (variant == ST_VARIANT_FAST) {
-        if (integ == 1) return launch_spin<VAR_FAST, INT_RK45, BATCH, true>;
-        return pulse ? launch_spin<VAR_FAST, INT_DOP853, BATCH, true> : launch_spin<VAR_FAST, INT_DOP853, BATCH, false>;
+template <int VAR, bool BATCH>
+static SpinFn pick_spin_v(int integ, bool pulse) {
+    switch (integ) {
+    case INT_RK45: return launch_spin<VAR, INT_RK45, BATCH, true>;
+    case INT_MAGNUS: return launch_spin<VAR, INT_MAGNUS, BATCH, true>;
+    case INT_RK45Q: return launch_spin<VAR, INT_RK45Q, BATCH, true>;
+    case INT_RKF45: return launch_spin<VAR, INT_RKF45, BATCH, true>;
+    case INT_RKF45Q: return launch_spin<VAR, INT_RKF45Q, BATCH, true>;
+    default:
+        if constexpr (VAR == VAR_FAST)
+            return pulse ? launch_spin<VAR, INT_DOP853, BATCH, true> : launch_spin<VAR, INT_DOP853, BATCH, false>;
+        else
+            return launch_spin<VAR, INT_DOP853, BATCH, true>;
     }
-    return integ == 1 ? launch_spin<VAR_EXACT, INT_RK45, BATCH, true> : launch_spin<VAR_EXACT, INT_DOP853, BATCH, true>;
 }
 static SpinFn pick_spin(int variant, int integ, bool batch, bool pulse) {
-    return batch ? pick_spin_b<true>(variant, integ, pulse) : pick_spin_b<false>(variant, integ, pulse);
+    if (variant == ST_VARIANT_FAST)
+        return batch ? pick_spin_v<VAR_FAST, true>(integ, pulse) : pick_spin_v<VAR_FAST, false>(integ, pulse);
+    return batch ? pick_spin_v<VAR_EXACT, true>(integ, pulse) : pick_spin_v<VAR_EXACT, false>(integ, pulse);
 }
-template <bool BATCH>
-static const void* spin_symbol_b(int variant, int integ, bool pulse) {
-    if (variant == ST_VARIANT_FAST) {
-        if (integ == 1) return (const void*)&k_spin<VAR_FAST, INT_RK45, BATCH, true>;
-        return pulse ? (const void*)&k_spin<VAR_FAST, INT_DOP853, BATCH, true>
-                     : (const void*)&k_spin_o3<VAR_FAST, INT_DOP853, BATCH, false>;
+template <int VAR, bool BATCH>
+static const void* spin_symbol_v(int integ, bool pulse) {
+    switch (integ) {
+    case INT_RK45: return (const void*)&k_spin<VAR, INT_RK45, BATCH, true>;
+    case INT_MAGNUS: return (const void*)&k_spin<VAR, INT_MAGNUS, BATCH, true>;
+    case INT_RK45Q: return (const void*)&k_spin<VAR, INT_RK45Q, BATCH, true>;
+    case INT_RKF45: return (const void*)&k_spin<VAR, INT_RKF45, BATCH, true>;
+    case INT_RKF45Q: return (const void*)&k_spin<VAR, INT_RKF45Q, BATCH, true>;
+    default:
+        if constexpr (VAR == VAR_FAST)
+            return pulse ? (const void*)&k_spin<VAR, INT_DOP853, BATCH, true>
+                         : (const void*)&k_spin_o3<VAR, INT_DOP853, BATCH, false>;
+        else
+            return (const void*)&k_spin<VAR, INT_DOP853, BATCH, true>;
     }
-    return integ == 1 ? (const void*)&k_spin<VAR_EXACT, INT_RK45, BATCH, true>
-                      : (const void*)&k_spin<VAR_EXACT, INT_DOP853, BATCH, true>;
 }
 static const void* spin_symbol(int variant, int integ, bool batch, bool pulse) {
-    return batch ? spin_symbol_b<true>(variant, integ, pulse) : spin_symbol_b<false>(variant, integ, pulse);
+    if (variant == ST_VARIANT_FAST)
+        return batch ? spin_symbol_v<VAR_FAST, true>(integ, pulse) : spin_symbol_v<VAR_FAST, false>(integ, pulse);
+    return batch ? spin_symbol_v<VAR_EXACT, true>(integ, pulse) : spin_symbol_v<VAR_EXACT, false>(integ, pulse);
 }
 
 static int persistent_grid(st_handle* h, const void* sym, int* cache) {
@@ -531,9 +548,18 @@ static int check_ext(const st_ext* ext, st_ext& x) {
 
 static int check_options(const st_options& o) {
     if (o.numParticles <= 0) return fail(nullptr, ST_E_ARG, "numParticles must be > 0 (got %d)", o.numParticles);
-    if (o.integratorType > 1 || o.integratorType < -1)
+    if (o.integratorType > 5 || o.integratorType < -1)
         return fail(nullptr, ST_E_UNSUPPORTED,
-                    "integratorType %d is not implemented (0 = DOP853, 1 = RK45, -1 = trajectory only)",
+                    "integratorType %d is not implemented (0 = DOP853, 1 = RK45, 2 = MagnusCFET, 3 = RK45 "
+                    "quaternion, 4 = RKF45, 5 = RKF45 quaternion, -1 = trajectory only)",
+                    o.integratorType);
+    return ST_OK;
+}
+
+// the fused engine carries DOP853 and RK45 only
+static int check_engine_integ(const st_options& o, const st_ext& x) {
+    if (x.engine == ST_ENGINE_FUSED && o.integratorType > 1)
+        return fail(nullptr, ST_E_UNSUPPORTED, "integratorType %d runs on the pipeline engine only",
                     o.integratorType);
     return ST_OK;
 }
@@ -578,6 +604,7 @@ ST_API int st_create(const st_options* opt, const st_ext* ext, st_handle** out) 
     if (r) return r;
     st_ext x;
     if ((r = check_ext(ext, x))) return r;
+    if ((r = check_engine_integ(*opt, x))) return r;
     if ((r = create_common(*opt, x, (uint32_t)opt->numParticles, out))) return r;
     // the spin kernel reads the physics block from device memory
     st_handle* h = *out;
@@ -767,6 +794,9 @@ ST_API int st_set_checkpointing(st_handle* h, int enable) {
     if (!h) return fail(nullptr, ST_E_ARG, "null handle");
     if (enable && h->ext.engine != ST_ENGINE_PIPELINE)
         return fail(h, ST_E_UNSUPPORTED, "checkpointing needs the pipeline engine");
+    if (enable && h->opt.integratorType != INT_DOP853)
+        return fail(h, ST_E_UNSUPPORTED, "the adjoint integrates the DOP853 path (integratorType 0); got %d",
+                    h->opt.integratorType);
     h->ckpt_on = enable != 0;
     return ST_OK;
 }

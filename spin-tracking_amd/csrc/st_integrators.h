@@ -575,6 +575,280 @@ ST_HD inline int rk_trial(const SimParams& P, const Phys& Fp, RkState& R, d3& y,
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// integratorTypes 2-5: integrateMagnusCFET (src/integrator.cpp:620-723),
+// integrateRK45Quaternion (:428-522), integrateRKF45 (:339-426) and
+// integrateRKF45Quaternion (:524-618).  Same protocol as rk_trial: one call is
+// one iteration of the reference's while(1), state in RkState (yy1 is used by
+// RKF45 only).  The arithmetic follows the reference's order in both variants;
+// the fast variant takes gamma*B from the segment polynomial.
+// ---------------------------------------------------------------------------
+namespace rkf {   // include/coeff.h:220-256 (Fehlberg)
+constexpr double A2 = 1.0 / 4.0, A3 = 3.0 / 8.0, A4 = 12.0 / 13.0, A5 = 1.0, A6 = 0.5;
+constexpr double B21 = 0.25;
+constexpr double B31 = 3.0 / 32.0, B32 = 9.0 / 32.0;
+constexpr double B41 = 1932.0 / 2197.0, B42 = -7200.0 / 2197.0, B43 = 7296.0 / 2197.0;
+constexpr double B51 = 439.0 / 216.0, B52 = -8.0, B53 = 3680.0 / 513.0, B54 = -845.0 / 4104.0;
+constexpr double B61 = -8.0 / 27.0, B62 = 2.0, B63 = -3544.0 / 2565.0, B64 = 1859.0 / 4104.0,
+                 B65 = -11.0 / 40.0;
+// fifth-order weights (the reference's C1..C6)
+constexpr double W1 = 16.0 / 135.0, W2 = 0.0, W3 = 6656.0 / 12825.0, W4 = 28561.0 / 56430.0,
+                 W5 = -9.0 / 50.0, W6 = 2.0 / 55.0;
+// the fourth-order weights, which the reference's error uses directly (:398)
+constexpr double R1 = 25.0 / 216.0, R2 = 0.0, R3 = 1408.0 / 2565.0, R4 = 2197.0 / 4104.0,
+                 R5 = -0.2, R6 = 0.0;
+}  // namespace rkf
+
+// Quaternions (src/quaternion.cpp:128-164)
+struct Quat {
+    double w, x, y, z;
+};
+
+ST_HD __forceinline__ Quat q_mult(const Quat& a, const Quat& b) {
+    return Quat{a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z, a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y,
+                a.w * b.y + a.y * b.w + a.z * b.x - a.x * b.z, a.w * b.z + a.z * b.w + a.x * b.y - a.y * b.x};
+}
+
+// q v q* (:137-148)
+ST_HD __forceinline__ d3 qv_mult(const Quat& q, const d3& v) {
+    const Quat r = q_mult(q_mult(q, Quat{0.0, v.x, v.y, v.z}), Quat{q.w, -q.x, -q.y, -q.z});
+    return d3{r.x, r.y, r.z};
+}
+
+// rotation by |k| dt about -k/|k| (:150-156)
+ST_HD __forceinline__ Quat rodriguez_quat(const d3& k, double dt) {
+    const double angle = len3(k);
+    const double nx = k.x / angle, ny = k.y / angle, nz = k.z / angle;
+    const double hh = angle * dt;
+    const double s = -sin(hh / 2.0);
+    return Quat{cos(hh / 2.0), nx * s, ny * s, nz * s};
+}
+
+// v rotated by |k| about -k/|k| (:158-163)
+ST_HD __forceinline__ d3 rodriguez(const d3& k, const d3& v) {
+    const double angle = len3(k);
+    const double s = sin(angle);
+    const double c = cos(angle);
+    const d3 vk = cross3(v, k);
+    const double sa = s / angle;
+    const double f = ((k.x * v.x + k.y * v.y + k.z * v.z) * (1.0 - c)) / (angle * angle);
+    return d3{(v.x * c + vk.x * sa) + k.x * f, (v.y * c + vk.y * sa) + k.y * f, (v.z * c + vk.z * sa) + k.z * f};
+}
+
+// Commutator-free exponential integrators of orders 8 and 6 (Alvermann &
+// Fehske, J. Comput. Phys. 230 (2011) 5930), gamma*B sampled at the 5
+// Gauss-Legendre nodes; include/coeff.h:258-321 (quadrature weights folded in).
+namespace cfet {
+constexpr double X1 = 0.046910077030668074, X2 = 0.2307653449471585, X3 = 0.5, X4 = 0.7692346550528415,
+                 X5 = 0.9530899229693319;
+}  // namespace cfet
+
+// (c1 B1 + c2 B2 + c3 B3 + c4 B4 + c5 B5) * h, summed left to right
+ST_HD __forceinline__ d3 cfet_exp(double c1, double c2, double c3, double c4, double c5, const d3* B, double h) {
+    return d3{((((c1 * B[0].x + c2 * B[1].x) + c3 * B[2].x) + c4 * B[3].x) + c5 * B[4].x) * h,
+              ((((c1 * B[0].y + c2 * B[1].y) + c3 * B[2].y) + c4 * B[3].y) + c5 * B[4].y) * h,
+              ((((c1 * B[0].z + c2 * B[1].z) + c3 * B[2].z) + c4 * B[3].z) + c5 * B[4].z) * h};
+}
+
+// the coefficient rows G1..G6 (order 8) and H1..H3 (order 6)
+#define ST_G1 -0.005984812750244602, 0.020217897716672675, -0.03646660716686794, 0.04245408899634467, 0.14949496424802844
+#define ST_G1R 0.14949496424802844, 0.04245408899634467, -0.03646660716686794, 0.020217897716672675, -0.005984812750244602
+#define ST_G2 0.02512881778391826, -0.08955337580937157, 0.19032828600393248, 0.28920194841768715, -0.03568486888016069
+#define ST_G2R -0.03568486888016069, 0.28920194841768715, 0.19032828600393248, -0.08955337580937157, 0.02512881778391826
+#define ST_G3 -0.04972613261435144, 0.6572496090804754, 0.11725149008224456, -0.43611911073577403, 0.18080345083145516
+#define ST_G3R 0.18080345083145516, -0.43611911073577403, 0.11725149008224456, 0.6572496090804754, -0.04972613261435144
+#define ST_G4 0.0464837870954061, -0.6265183532355708, -0.11144873822286155, 0.4154988625423615, -0.17224148557040517
+#define ST_G4R -0.17224148557040517, 0.4154988625423615, -0.11144873822286155, -0.6265183532355708, 0.0464837870954061
+#define ST_G5 0.015567390500242426, -0.0677120111619075, -0.15241336773272682, -0.08803798509100481, -0.0013284996209207986
+#define ST_G5R -0.0013284996209207986, -0.08803798509100481, -0.15241336773272682, -0.0677120111619075, 0.015567390500242426
+#define ST_G6 -0.034049168494873015, 0.12263276452977083, 0.2699423185170027, 0.12263276452977094, -0.03404916849487304
+#define ST_H1 -0.007431519425246311, 0.022452824849980513, -0.03620393417193995, 0.0409779901800809, 0.1516046385671249
+#define ST_H1R 0.1516046385671249, 0.0409779901800809, -0.03620393417193995, 0.022452824849980513, -0.007431519425246311
+#define ST_H2 0.03095749686312404, -0.10116885619385878, 0.1997455955716367, 0.3010242365525433, -0.055594729593982636
+#define ST_H2R -0.055594729593982636, 0.3010242365525433, 0.1997455955716367, -0.10116885619385878, 0.03095749686312404
+#define ST_H3 -0.001072443882925378, -0.023971860139062367, -0.04263887835494916, -0.023971860139062388, \
+              -0.0010724438829253797
+
+// Step-size bounds of the four (e.g. :355-385): the end-of-interval test
+// comes first and skips the hmax/hmin checks, and a fixed step size never
+// sets stop (such a call ends only at nmax, with -2, as in the reference).
+ST_HD __forceinline__ int bounds_v2(const SimParams& P, RkState& R, double& h) {
+    R.nstep++;
+    if ((uint32_t)R.nstep > P.nmax) return -2;
+    const double endDt = R.tf - R.t;
+    R.lastH = h;
+    if (P.fixed) {
+        h = P.h0;
+    } else if (h >= endDt) {
+        h = endDt;
+        R.stop = true;
+    } else {
+        R.lastH = h;
+        R.stop = false;
+        if (h > P.hmax) h = std_min(h, P.hmax);
+        else if (h < P.hmin) return -1;
+    }
+    return 0;
+}
+
+// Controller and accept test (e.g. :400-424); 1 when the call ends.
+ST_HD __forceinline__ int control_v2(const SimParams& P, RkState& R, d3& y, const d3& ynew, double& h,
+                                     double error, uint32_t& nacc, uint32_t& nrej) {
+    error = std_max(error, 1.0E-16);
+    const double tol = P.rtol;
+    const double ratio = tol / error;
+    double q_ = pow(ratio, 0.7 / 7.0) * pow(R.prev_ratio, -0.4 / 7.0);
+    q_ = std_min(q_, 4.0);
+    if ((q_ > 0.81 && error < 2 * tol) || P.fixed) {
+        nacc++;
+        y = ynew;
+        R.t += h;
+        if (R.stop) {
+            h = R.lastH;
+            return 1;
+        }
+    } else {
+        nrej++;
+        if (R.stop) R.stop = false;
+    }
+    if (!P.fixed) h = h * q_;
+    R.prev_ratio = ratio;
+    return 0;
+}
+
+template <int VAR, int INTEG>
+ST_HD inline int rk2_trial(const SimParams& P, const Phys& Fp, RkState& R, d3& y, double& h, const Seg& S,
+                           const Poly& q, uint32_t& nacc, uint32_t& nrej) {
+    auto W = [&](double tt) -> d3 {
+        if (VAR == VAR_EXACT) return omega_exact(P, Fp, tt, S);
+        return omega_fast(tt, S.t0, q);
+    };
+    {
+        const int b = bounds_v2(P, R, h);
+        if (b) return b;
+    }
+    const double t = R.t;
+    if constexpr (INTEG == INT_RKF45) {
+        using namespace rkf;
+        auto F = [&](double tt, const d3& yy) -> d3 { return cross3(yy, W(tt)); };
+        d3 yy1 = R.yy1;
+        const d3 k1 = F(t, yy1);
+        {
+            const double c = h * B21;
+            ST_V3(yy1, y.x + c * k1.x, y.y + c * k1.y, y.z + c * k1.z);
+        }
+        const d3 k2 = F(t + A2 * h, yy1);
+        {
+            const double c1 = h * B31, c2_ = h * B32;
+            ST_V3(yy1, y.x + c1 * k1.x + c2_ * k2.x, y.y + c1 * k1.y + c2_ * k2.y, y.z + c1 * k1.z + c2_ * k2.z);
+        }
+        const d3 k3 = F(t + A3 * h, yy1);
+        {
+            const double c1 = h * B41, c2_ = h * B42, c3_ = h * B43;
+            ST_V3(yy1, y.x + c1 * k1.x + c2_ * k2.x + c3_ * k3.x, y.y + c1 * k1.y + c2_ * k2.y + c3_ * k3.y,
+                  y.z + c1 * k1.z + c2_ * k2.z + c3_ * k3.z);
+        }
+        const d3 k4 = F(t + A4 * h, yy1);
+        {
+            const double c1 = h * B51, c2_ = h * B52, c3_ = h * B53, c4_ = h * B54;
+            ST_V3(yy1, y.x + c1 * k1.x + c2_ * k2.x + c3_ * k3.x + c4_ * k4.x,
+                  y.y + c1 * k1.y + c2_ * k2.y + c3_ * k3.y + c4_ * k4.y,
+                  y.z + c1 * k1.z + c2_ * k2.z + c3_ * k3.z + c4_ * k4.z);
+        }
+        const d3 k5 = F(t + A5 * h, yy1);
+        {
+            const double c1 = h * B61, c2_ = h * B62, c3_ = h * B63, c4_ = h * B64, c5_ = h * B65;
+            ST_V3(yy1, y.x + c1 * k1.x + c2_ * k2.x + c3_ * k3.x + c4_ * k4.x + c5_ * k5.x,
+                  y.y + c1 * k1.y + c2_ * k2.y + c3_ * k3.y + c4_ * k4.y + c5_ * k5.y,
+                  y.z + c1 * k1.z + c2_ * k2.z + c3_ * k3.z + c4_ * k4.z + c5_ * k5.z);
+        }
+        const d3 k6 = F(t + A6 * h, yy1);
+        R.yy1 = yy1;
+        d3 ws, te;
+        ST_V3(ws, y.x + h * (k1.x * W1 + k2.x * W2 + k3.x * W3 + k4.x * W4 + k5.x * W5 + k6.x * W6),
+              y.y + h * (k1.y * W1 + k2.y * W2 + k3.y * W3 + k4.y * W4 + k5.y * W5 + k6.y * W6),
+              y.z + h * (k1.z * W1 + k2.z * W2 + k3.z * W3 + k4.z * W4 + k5.z * W5 + k6.z * W6));
+        ST_V3(te, h * (R1 * k1.x + R2 * k2.x + R3 * k3.x + R4 * k4.x + R5 * k5.x + R6 * k6.x),
+              h * (R1 * k1.y + R2 * k2.y + R3 * k3.y + R4 * k4.y + R5 * k5.y + R6 * k6.y),
+              h * (R1 * k1.z + R2 * k2.z + R3 * k3.z + R4 * k4.z + R5 * k5.z + R6 * k6.z));
+        return control_v2(P, R, y, ws, h, len3(te), nacc, nrej);
+    } else if constexpr (INTEG == INT_RK45Q || INTEG == INT_RKF45Q) {
+        // k_i = gamma*B(t + A_i h); the step rotates y by the product of
+        // R(k_i, h W_i), k1 first; the comparison solution drops the sixth
+        // factor and uses weights E_i (RK45: W = CH, E = C; RKF45: W = E = C)
+        constexpr bool F45 = INTEG == INT_RKF45Q;
+        constexpr double A[6] = {0.0, F45 ? rkf::A2 : rk::A2, F45 ? rkf::A3 : rk::A3, F45 ? rkf::A4 : rk::A4,
+                                 F45 ? rkf::A5 : rk::A5, F45 ? rkf::A6 : rk::A6};
+        constexpr double Wt[6] = {F45 ? rkf::W1 : rk::CH1, F45 ? rkf::W2 : rk::CH2, F45 ? rkf::W3 : rk::CH3,
+                                  F45 ? rkf::W4 : rk::CH4, F45 ? rkf::W5 : rk::CH5, F45 ? rkf::W6 : rk::CH6};
+        constexpr double Et[5] = {F45 ? rkf::W1 : 1.0 / 9.0, F45 ? rkf::W2 : 0.0, F45 ? rkf::W3 : 9.0 / 20.0,
+                                  F45 ? rkf::W4 : 16.0 / 45.0, F45 ? rkf::W5 : 1.0 / 12.0};
+        d3 k = W(t);
+        Quat qw = rodriguez_quat(k, h * Wt[0]);
+        Quat qe = rodriguez_quat(k, h * Et[0]);
+#pragma unroll
+        for (int i = 1; i < 6; ++i) {
+            k = W(t + A[i] * h);
+            qw = q_mult(rodriguez_quat(k, h * Wt[i]), qw);
+            if (i < 5) qe = q_mult(rodriguez_quat(k, h * Et[i]), qe);
+        }
+        const d3 ws = qv_mult(qw, y);
+        const d3 te = qv_mult(qe, y);
+        const d3 d{ws.x - te.x, ws.y - te.y, ws.z - te.z};
+        return control_v2(P, R, y, ws, h, len3(d), nacc, nrej);
+    } else {
+        static_assert(INTEG == INT_MAGNUS, "integrator type");
+        using namespace cfet;
+        const d3 B[5] = {W(t + X1 * h), W(t + X2 * h), W(t + X3 * h), W(t + X4 * h), W(t + X5 * h)};
+        // order 8: k11 (row 1 reversed) acts first ... k6 (row 6) ... k1 (row 1)
+        d3 y8 = rodriguez(cfet_exp(ST_G1R, B, h), y);
+        y8 = rodriguez(cfet_exp(ST_G2R, B, h), y8);
+        y8 = rodriguez(cfet_exp(ST_G3R, B, h), y8);
+        y8 = rodriguez(cfet_exp(ST_G4R, B, h), y8);
+        y8 = rodriguez(cfet_exp(ST_G5R, B, h), y8);
+        y8 = rodriguez(cfet_exp(ST_G6, B, h), y8);
+        y8 = rodriguez(cfet_exp(ST_G5, B, h), y8);
+        y8 = rodriguez(cfet_exp(ST_G4, B, h), y8);
+        y8 = rodriguez(cfet_exp(ST_G3, B, h), y8);
+        y8 = rodriguez(cfet_exp(ST_G2, B, h), y8);
+        y8 = rodriguez(cfet_exp(ST_G1, B, h), y8);
+        // order 6: k5 (row 1 reversed), k4 (row 2 reversed), k3, k2, k1
+        d3 y6 = rodriguez(cfet_exp(ST_H1R, B, h), y);
+        y6 = rodriguez(cfet_exp(ST_H2R, B, h), y6);
+        y6 = rodriguez(cfet_exp(ST_H3, B, h), y6);
+        y6 = rodriguez(cfet_exp(ST_H2, B, h), y6);
+        y6 = rodriguez(cfet_exp(ST_H1, B, h), y6);
+        const d3 d{y8.x - y6.x, y8.y - y6.y, y8.z - y6.z};
+        return control_v2(P, R, y, y8, h, len3(d), nacc, nrej);
+    }
+}
+
+// every non-DOP853 integrator behind one call
+template <int VAR, int INTEG>
+ST_HD __forceinline__ int rk_any_trial(const SimParams& P, const Phys& Fp, RkState& R, d3& y, double& h,
+                                       const Seg& S, const Poly& q, uint32_t& nacc, uint32_t& nrej) {
+    if constexpr (INTEG == INT_RK45)
+        return rk_trial<VAR>(P, Fp, R, y, h, S, q, nacc, nrej);
+    else
+        return rk2_trial<VAR, INTEG>(P, Fp, R, y, h, S, q, nacc, nrej);
+}
+
+#undef ST_G1
+#undef ST_G1R
+#undef ST_G2
+#undef ST_G2R
+#undef ST_G3
+#undef ST_G3R
+#undef ST_G4
+#undef ST_G4R
+#undef ST_G5
+#undef ST_G5R
+#undef ST_G6
+#undef ST_H1
+#undef ST_H1R
+#undef ST_H2
+#undef ST_H2R
+#undef ST_H3
 #undef ST_V3
 
 }  // namespace st

@@ -412,11 +412,11 @@ __device__ __forceinline__ void spin_body(const SimParams& P, const DevState& D,
         }
         if (phase == S_INT) {
             int st;
-            if (INTEG == INT_DOP853) {
+            if constexpr (INTEG == INT_DOP853) {
                 if (VAR == VAR_EXACT) st = dop_trial_exact(P, phys(), dop, L.S, L.tempH, seg, L.acc, L.rej);
                 else st = dop_trial_fast<PULSE>(P, phys(), dop, L.S, L.tempH, seg, poly, L.acc, L.rej);
             } else {
-                st = rk_trial<VAR>(P, phys(), rk, L.S, L.tempH, seg, poly, L.acc, L.rej);
+                st = rk_any_trial<VAR, INTEG>(P, phys(), rk, L.S, L.tempH, seg, poly, L.acc, L.rej);
             }
             if (st != 0) {
                 if (R.trace && L.pid == R.trace_pid)

@@ -37,8 +37,17 @@ def test_error_path_without_device_call():
     assert lib.st_create(C.byref(o), None, C.byref(h)) == _lib.ST_E_ARG
     assert b"numParticles" in lib.st_last_error(None)
     o.numParticles = 10
-    o.integratorType = 2
+    # integratorType 6 (the spectral path) is not carried
+    o.integratorType = 6
     assert lib.st_create(C.byref(o), None, C.byref(h)) == _lib.ST_E_UNSUPPORTED
+    assert b"integratorType 6" in lib.st_last_error(None)
+    # types 2-5 run on the pipeline engine; the fused engine refuses them
+    # before touching the device
+    o.integratorType = 2
+    x = _lib.Ext()
+    x.pulse_axis, x.variant, x.engine = 2, _lib.VARIANT_EXACT, _lib.ENGINE_FUSED
+    assert lib.st_create(C.byref(o), C.byref(x), C.byref(h)) == _lib.ST_E_UNSUPPORTED
+    assert b"pipeline engine" in lib.st_last_error(None)
     assert lib.st_advance_to(None, 1.0) == _lib.ST_E_ARG
 
 

@@ -68,16 +68,17 @@ def check_integer_state(O, g, gstate, gcnt, E):
     return r
 
 
-def compare(O, g, gstate, gcnt, E, tol=TOL):
+def compare(O, g, gstate, gcnt, E, tol=TOL, step_slack=0):
     r = check_integer_state(O, g, gstate, gcnt, E)
     n_ode = E.n_ode.reshape(-1, 3)
     out = {}
     # accepted/rejected trial counts follow the step-size controller, which
     # sees last-bit libm differences (pow in the controller, log in the
-    # initial normal draws); reported, and bounded loosely
+    # initial normal draws); reported, and bounded loosely (step_slack more
+    # steps for integrators whose error estimate sits at round-off level)
     da = np.abs(gcnt["accepted"].astype(np.int64) - n_ode[:, 0].astype(np.int64))
     out["steps_identical"] = float(np.mean(da == 0))
-    assert (da / np.maximum(1, n_ode[:, 0])).max() < 0.05
+    assert ((da - step_slack) / np.maximum(1, n_ode[:, 0])).max() < 0.05
     for k in ("S", "pos", "v"):
         a, b = st.xyz(g[k]), O.xyz(r[k])
         assert np.array_equal(np.isfinite(a), np.isfinite(b)), f"{k} NaN pattern"
