@@ -4,6 +4,7 @@
 #include <cstring>
 #include <iostream>
 #include <memory>
+#include <stdexcept>
 
 #include "st_host.h"
 
@@ -54,7 +55,7 @@ int report(st_handle* h, int rc, const char* what) {
 
 // particle::outputData (src/particle.cpp:852-862): copy the eight arrays out
 // and hand them to the logger
-int output_data(st_handle* h, const st_options& opt, BinaryLogger& log) {
+int output_data(st_handle* h, const st_options& opt, Logger& log) {
     const size_t n = (size_t)opt.numParticles;
     std::vector<double> t(n);
     std::vector<st_coords> pos(n), v(n), S(n);
@@ -76,7 +77,14 @@ int main_analysis(st_options opt, const char* output, const RunConfig& cfg, std:
     int rc = st_create(&opt, &cfg.ext, &H.h);
     if (rc) return report(nullptr, rc, "st_create");
     if ((rc = st_init(H.h))) return report(H.h, rc, "st_init");
-    BinaryLogger log(opt, output);
+    std::unique_ptr<Logger> logp;
+    try {
+        logp = createLogger(opt, output);
+    } catch (const std::exception& e) {
+        std::cerr << e.what() << "\n";
+        return ST_E_ARG;
+    }
+    Logger& log = *logp;
     if (!log.ok()) {
         std::cerr << "cannot open " << output << "\n";
         return ST_E_ARG;
@@ -96,6 +104,7 @@ int main_analysis(st_options opt, const char* output, const RunConfig& cfg, std:
         if ((rc = st_memcpy_d2h(H.h, host.data(), blocks.p, host.size()))) return report(H.h, rc, "st_memcpy_d2h");
         const auto ms = std::chrono::duration_cast<std::chrono::milliseconds>(clk::now() - t0).count();
         for (size_t k = 0; k < times.size(); ++k) log.writeBlock(host.data() + k * bb, bb);
+        if (!log.ok()) return report(nullptr, ST_E_ARG, "writing the output file");
         out << "run " << times.size() << " intervals, duration " << (times.empty() ? 0.0 : times.back()) << ", "
             << ms << std::endl;
         return ST_OK;
@@ -104,6 +113,7 @@ int main_analysis(st_options opt, const char* output, const RunConfig& cfg, std:
         const auto t0 = clk::now();
         if ((rc = st_advance_to(H.h, times[i]))) return report(H.h, rc, "st_advance_to");
         if ((rc = output_data(H.h, opt, log))) return rc;
+        if (!log.ok()) return report(nullptr, ST_E_ARG, "writing the output file");
         const auto ms = std::chrono::duration_cast<std::chrono::milliseconds>(clk::now() - t0).count();
         out << "iter " << i << ", duration " << times[i] << ", " << ms << std::endl;
     }

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <iosfwd>
 #include <string>
+#include <memory>
 #include <vector>
 
 #include "../../include/spintrack.h"
@@ -33,25 +34,48 @@ void derive(st_options& opt);
 // floor((tf - t0)/ioutInt) output times (i+1)*ioutInt (src/simulation.cpp:79-85).
 std::vector<double> output_times(const st_options& opt);
 
+// Logger (include/logger.h:12-30): the options, then one snapshot per
+// output time.
+class Logger {
+public:
+    virtual ~Logger() = default;
+    virtual bool ok() const = 0;
+    virtual int32_t numParticles() const = 0;
+    virtual void writeOptions() = 0;
+    virtual void writeSnapshot(const double* t, const st_coords* pos, const st_coords* v, const st_coords* S,
+                               const int32_t* failure, const uint64_t* n_coll, const uint64_t* n_bounce,
+                               const uint64_t* n_steps) = 0;
+    // a block in the binary file layout (st_snapshot_block_bytes bytes:
+    // t, pos, v, S, failure, n_coll, n_bounce, n_steps); by default split
+    // into its arrays and written as a snapshot
+    virtual void writeBlock(const void* block, size_t bytes);
+};
+
 // BinaryLogger (src/logger.cpp:8-38): the 440-byte options block, then one
 // snapshot block per output -- t, pos, v, S, failure, n_coll, n_bounce,
 // n_steps, numParticles entries each.
-class BinaryLogger {
+class BinaryLogger : public Logger {
 public:
     BinaryLogger(const st_options& opt, const char* path);
-    ~BinaryLogger();
-    bool ok() const { return f_ != nullptr; }
-    void writeOptions();
+    ~BinaryLogger() override;
+    bool ok() const override { return f_ != nullptr; }
+    int32_t numParticles() const override { return opt_.numParticles; }
+    void writeOptions() override;
     void writeSnapshot(const double* t, const st_coords* pos, const st_coords* v, const st_coords* S,
                        const int32_t* failure, const uint64_t* n_coll, const uint64_t* n_bounce,
-                       const uint64_t* n_steps);
-    // a block already in the file layout (st_snapshot_block_bytes bytes)
-    void writeBlock(const void* block, size_t bytes);
+                       const uint64_t* n_steps) override;
+    // the block is already the file's bytes
+    void writeBlock(const void* block, size_t bytes) override;
 
 private:
     st_options opt_;
     FILE* f_ = nullptr;
 };
+
+// createLogger (src/logger.cpp:190-207): HDF5Logger for a ".h5" / ".hdf5"
+// path (any case), BinaryLogger otherwise.  Throws std::runtime_error if
+// HDF5 output is asked for and this build or host has no libhdf5.
+std::unique_ptr<Logger> createLogger(const st_options& opt, const char* path);
 
 struct RunConfig {
     st_ext ext{};
