@@ -306,9 +306,16 @@ static const void* fused_symbol(int variant, int integ) {
 // Spin kernels.  The fast DOP853 kernel has a form without the pulse term
 // (PULSE = false: fewer live registers) for handles whose configurations
 // have no pulse; the other kernels evaluate the pulse switch at run time.
+// The fast DOP853 kernels run at occupancy 3 (the pulse form spilling ~90 B
+// per lane to do so), except the batched pulse form, whose per-lane pulse
+// fields make the spills cost more than the third wave gains.
+template <int VAR, int INTEG, bool BATCH, bool PULSE>
+constexpr bool use_o3() {
+    return VAR == VAR_FAST && INTEG == INT_DOP853 && !(BATCH && PULSE);
+}
 template <int VAR, int INTEG, bool BATCH, bool PULSE>
 static hipError_t launch_spin(st_handle* h, dim3 grid, const PipeArgs& A) {
-    if constexpr (VAR == VAR_FAST && INTEG == INT_DOP853 && !PULSE)
+    if constexpr (use_o3<VAR, INTEG, BATCH, PULSE>())
         hipLaunchKernelGGL((k_spin_o3<VAR, INTEG, BATCH, PULSE>), grid, dim3(256), 0, h->stream, h->P, h->D, A);
     else
         hipLaunchKernelGGL((k_spin<VAR, INTEG, BATCH, PULSE>), grid, dim3(256), 0, h->stream, h->P, h->D, A);
@@ -345,7 +352,9 @@ static const void* spin_symbol_v(int integ, bool pulse) {
     case INT_RKF45Q: return (const void*)&k_spin<VAR, INT_RKF45Q, BATCH, true>;
     default:
         if constexpr (VAR == VAR_FAST)
-            return pulse ? (const void*)&k_spin<VAR, INT_DOP853, BATCH, true>
+            return pulse ? (use_o3<VAR, INT_DOP853, BATCH, true>()
+                                ? (const void*)&k_spin_o3<VAR, INT_DOP853, BATCH, true>
+                                : (const void*)&k_spin<VAR, INT_DOP853, BATCH, true>)
                          : (const void*)&k_spin_o3<VAR, INT_DOP853, BATCH, false>;
         else
             return (const void*)&k_spin<VAR, INT_DOP853, BATCH, true>;

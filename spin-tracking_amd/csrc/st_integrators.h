@@ -669,6 +669,24 @@ ST_HD __forceinline__ d3 cfet_exp(double c1, double c2, double c3, double c4, do
 #define ST_H3 -0.001072443882925378, -0.023971860139062367, -0.04263887835494916, -0.023971860139062388, \
               -0.0010724438829253797
 
+// The 16 CFET exponents in the order they act (coefficients of B1..B5): the
+// order-8 chain k11 .. k1 (:688-700), then the order-6 chain k5 .. k1
+// (:702-708).  Read with a uniform index by a rolled loop: one copy of the
+// rotation code instead of sixteen keeps the kernel's registers in check.
+__constant__ double CFET_ROWS[16][5] = {{ST_G1R}, {ST_G2R}, {ST_G3R}, {ST_G4R}, {ST_G5R}, {ST_G6},
+                                        {ST_G5},  {ST_G4},  {ST_G3},  {ST_G2},  {ST_G1}, {ST_H1R},
+                                        {ST_H2R}, {ST_H3},  {ST_H2},  {ST_H1}};
+
+// Quaternion integrators, per stage i = 0..5: node A_i, weight W_i of the
+// step, weight E_i of the comparison solution (E_5 unused); [0] RK45
+// (include/coeff.h:178-218: A, CH, C), [1] RKF45 (:220-256: A, C, C).
+__constant__ double QUAT_TAB[2][6][3] = {
+    {{0.0, 47.0 / 450.0, 1.0 / 9.0}, {2.0 / 9.0, 0.0, 0.0}, {1.0 / 3.0, 12.0 / 25.0, 9.0 / 20.0},
+     {3.0 / 4.0, 32.0 / 225.0, 16.0 / 45.0}, {1.0, 1.0 / 30.0, 1.0 / 12.0}, {5.0 / 6.0, 6.0 / 25.0, 0.0}},
+    {{0.0, 16.0 / 135.0, 16.0 / 135.0}, {1.0 / 4.0, 0.0, 0.0}, {3.0 / 8.0, 6656.0 / 12825.0, 6656.0 / 12825.0},
+     {12.0 / 13.0, 28561.0 / 56430.0, 28561.0 / 56430.0}, {1.0, -9.0 / 50.0, -9.0 / 50.0},
+     {0.5, 2.0 / 55.0, 0.0}}};
+
 // Step-size bounds of the four (e.g. :355-385): the end-of-interval test
 // comes first and skips the hmax/hmin checks, and a fixed step size never
 // sets stop (such a call ends only at nmax, with -2, as in the reference).
@@ -776,21 +794,15 @@ ST_HD inline int rk2_trial(const SimParams& P, const Phys& Fp, RkState& R, d3& y
         // k_i = gamma*B(t + A_i h); the step rotates y by the product of
         // R(k_i, h W_i), k1 first; the comparison solution drops the sixth
         // factor and uses weights E_i (RK45: W = CH, E = C; RKF45: W = E = C)
-        constexpr bool F45 = INTEG == INT_RKF45Q;
-        constexpr double A[6] = {0.0, F45 ? rkf::A2 : rk::A2, F45 ? rkf::A3 : rk::A3, F45 ? rkf::A4 : rk::A4,
-                                 F45 ? rkf::A5 : rk::A5, F45 ? rkf::A6 : rk::A6};
-        constexpr double Wt[6] = {F45 ? rkf::W1 : rk::CH1, F45 ? rkf::W2 : rk::CH2, F45 ? rkf::W3 : rk::CH3,
-                                  F45 ? rkf::W4 : rk::CH4, F45 ? rkf::W5 : rk::CH5, F45 ? rkf::W6 : rk::CH6};
-        constexpr double Et[5] = {F45 ? rkf::W1 : 1.0 / 9.0, F45 ? rkf::W2 : 0.0, F45 ? rkf::W3 : 9.0 / 20.0,
-                                  F45 ? rkf::W4 : 16.0 / 45.0, F45 ? rkf::W5 : 1.0 / 12.0};
+        const double(*Q)[3] = QUAT_TAB[INTEG == INT_RKF45Q ? 1 : 0];
         d3 k = W(t);
-        Quat qw = rodriguez_quat(k, h * Wt[0]);
-        Quat qe = rodriguez_quat(k, h * Et[0]);
-#pragma unroll
+        Quat qw = rodriguez_quat(k, h * Q[0][1]);
+        Quat qe = rodriguez_quat(k, h * Q[0][2]);
+#pragma unroll 1
         for (int i = 1; i < 6; ++i) {
-            k = W(t + A[i] * h);
-            qw = q_mult(rodriguez_quat(k, h * Wt[i]), qw);
-            if (i < 5) qe = q_mult(rodriguez_quat(k, h * Et[i]), qe);
+            k = W(t + Q[i][0] * h);
+            qw = q_mult(rodriguez_quat(k, h * Q[i][1]), qw);
+            if (i < 5) qe = q_mult(rodriguez_quat(k, h * Q[i][2]), qe);
         }
         const d3 ws = qv_mult(qw, y);
         const d3 te = qv_mult(qe, y);
@@ -800,24 +812,18 @@ ST_HD inline int rk2_trial(const SimParams& P, const Phys& Fp, RkState& R, d3& y
         static_assert(INTEG == INT_MAGNUS, "integrator type");
         using namespace cfet;
         const d3 B[5] = {W(t + X1 * h), W(t + X2 * h), W(t + X3 * h), W(t + X4 * h), W(t + X5 * h)};
-        // order 8: k11 (row 1 reversed) acts first ... k6 (row 6) ... k1 (row 1)
-        d3 y8 = rodriguez(cfet_exp(ST_G1R, B, h), y);
-        y8 = rodriguez(cfet_exp(ST_G2R, B, h), y8);
-        y8 = rodriguez(cfet_exp(ST_G3R, B, h), y8);
-        y8 = rodriguez(cfet_exp(ST_G4R, B, h), y8);
-        y8 = rodriguez(cfet_exp(ST_G5R, B, h), y8);
-        y8 = rodriguez(cfet_exp(ST_G6, B, h), y8);
-        y8 = rodriguez(cfet_exp(ST_G5, B, h), y8);
-        y8 = rodriguez(cfet_exp(ST_G4, B, h), y8);
-        y8 = rodriguez(cfet_exp(ST_G3, B, h), y8);
-        y8 = rodriguez(cfet_exp(ST_G2, B, h), y8);
-        y8 = rodriguez(cfet_exp(ST_G1, B, h), y8);
-        // order 6: k5 (row 1 reversed), k4 (row 2 reversed), k3, k2, k1
-        d3 y6 = rodriguez(cfet_exp(ST_H1R, B, h), y);
-        y6 = rodriguez(cfet_exp(ST_H2R, B, h), y6);
-        y6 = rodriguez(cfet_exp(ST_H3, B, h), y6);
-        y6 = rodriguez(cfet_exp(ST_H2, B, h), y6);
-        y6 = rodriguez(cfet_exp(ST_H1, B, h), y6);
+        // order 8: k11 (row 1 reversed) acts first ... k6 (row 6) ... k1
+        // (row 1); then order 6: k5 (row 1 reversed), k4 (row 2 reversed),
+        // k3, k2, k1 -- each chain starting from y
+        d3 y8 = y, y6 = y;
+#pragma unroll 1
+        for (int r = 0; r < 16; ++r) {
+            const double* c = CFET_ROWS[r];
+            const d3 k = cfet_exp(c[0], c[1], c[2], c[3], c[4], B, h);
+            const d3 out = rodriguez(k, r < 11 ? y8 : y6);
+            if (r < 11) y8 = out;
+            else y6 = out;
+        }
         const d3 d{y8.x - y6.x, y8.y - y6.y, y8.z - y6.z};
         return control_v2(P, R, y, y8, h, len3(d), nacc, nrej);
     }
