@@ -224,6 +224,33 @@ def bench_rk45(args, variant):
             "spin_TFLOPs": tf, "spin_frac_fp64_peak": tf / FP64_PEAK_TFLOPS}
 
 
+def bench_snapshot(args):
+    """The snapshot stream of the per-interval path: k_snapshot turns the
+    SoA particle state into one block of the reference's file layout (112 B
+    of state read and 108 B written per particle), coordinate fields staged
+    through LDS.  HIP events around 50 calls on the library's stream."""
+    import spintrack as st
+
+    n = args.particles
+    opt = build_workload(n, 0)
+    with st.Particle(opt, variant=st.VARIANT_FAST) as p:
+        p.initParticles()
+        bb = p.block_bytes()
+        buf = st._lib.DeviceBuffer(bb)
+        p.snapshot_block_device(buf.ptr)
+        p.synchronize()
+        reps = 50
+        p.timer_start()
+        for _ in range(reps):
+            p.snapshot_block_device(buf.ptr)
+        ms = p.timer_stop() / reps
+        buf.free()
+    nbytes = (112 + 108) * n
+    gbs = nbytes / (ms / 1e3) / 1e9
+    return {"kernel": "k_snapshot", "particles": n, "us_per_block": ms * 1e3, "algorithmic_bytes": nbytes,
+            "GBs": gbs, "frac_hbm_peak": gbs / HBM_PEAK_GBS}
+
+
 # integratorType -> (name, rtol, simulated seconds): tolerances each method
 # can meet above hmin = 1e-8 (RKF45's "error" is the step's whole increment,
 # RKF45Q's the sixth rotation factor)
@@ -592,6 +619,9 @@ def main():
         rk45 = bench_rk45(args, variant)
         print(f"[bench] rk45: {rk45}", file=sys.stderr, flush=True)
 
+    snap = bench_snapshot(args)
+    print(f"[bench] snapshot stream: {snap}", file=sys.stderr, flush=True)
+
     integ = None
     if not args.no_integrators:
         integ = bench_integrators(args, variant)
@@ -637,6 +667,7 @@ def main():
         "batched_sweep": sweep,
         "rk45": rk45,
         "integrators": integ,
+        "snapshot_stream": snap,
         "dressing": dressing,
         "adjoint": adj,
     }

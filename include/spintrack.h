@@ -76,7 +76,12 @@ typedef struct st_ext {
     int32_t seg_buffer_mb;  /* HBM budget of the pipeline's segment-record buffer; 0 = 16384 */
     int32_t chunk_records;  /* trajectory segments per particle per pipeline chunk (and between
                                adjoint checkpoints); 0 = as many as seg_buffer_mb holds */
-    int32_t reserved[2];
+    int32_t graph_chunks;   /* pipeline engine: capture this many chunks (rounded up to even, at
+                               most 64) in one HIP graph and let st_advance_to / st_run replay it
+                               until every particle is done -- one host round trip per replay
+                               instead of per chunk; 0 = host-driven chunk loop.  Runs with
+                               checkpointing or per-kernel timing on use the host-driven loop. */
+    int32_t reserved;
 } st_ext;
 
 /* include/particle.h:52-59 */

@@ -123,15 +123,60 @@ __global__ __launch_bounds__(256) void k_init(const SimParams P, const DevState 
     D.st<uint64_t>(F_CFG, o, cfg);
 }
 
-// Current state -> one snapshot block (reference file layout).
+// Graph-captured chunk loop: the counters a chunk uses live in small[]
+// ([0] spin queue head, [1] next-list length, [2] failures, [4] this chunk's
+// list length).  Single-thread kernels stand in for host memsets, which a
+// captured graph must not hold.
+__global__ void k_graph_start(uint32_t* small, uint32_t n) {
+    small[0] = 0;
+    small[1] = n;   // the "next list" of chunk -1: every particle, in list 0
+    small[2] = 0;
+    small[4] = 0;
+}
+// the list the previous chunk left becomes this chunk's; counters restart
+__global__ void k_chunk_head(uint32_t* small) {
+    small[4] = small[1];
+    small[0] = 0;
+    small[1] = 0;
+    small[2] = 0;
+}
+
+// Current state -> one snapshot block (reference file layout).  A workgroup
+// owns 256 consecutive particles.  t, failure and the counters are one
+// element per lane and go straight out (consecutive lanes, consecutive
+// addresses).  The coordinate fields are 24-byte {x, y, z} records: each is
+// staged in LDS as the byte image of the group's 6 KiB slice and written back
+// as contiguous 8-byte stores (2 KiB per store instruction across the group).
 __global__ __launch_bounds__(256) void k_snapshot(const DevState D, uint32_t n, char* B) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    __shared__ double stage[3 * 256];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t base = blockIdx.x * 256u;
+    const uint32_t i = base + tid;
+    const uint32_t cnt = min(256u, n - base);
+    const bool live = i < n;
     const uint32_t o = i * 8u;
-    write_record(B, n, i, D.ld(F_T, o), d3{D.ld(F_PX, o), D.ld(F_PY, o), D.ld(F_PZ, o)},
-                 d3{D.ld(F_VX, o), D.ld(F_VY, o), D.ld(F_VZ, o)}, d3{D.ld(F_SX, o), D.ld(F_SY, o), D.ld(F_SZ, o)},
-                 (int32_t)(uint32_t)D.ld<uint64_t>(F_FLAGS, o), D.ld<uint64_t>(F_NCOLL, o),
-                 D.ld<uint64_t>(F_NBOUNCE, o), D.ld<uint64_t>(F_NSTEPS, o));
+    const uint64_t N = n;
+    if (live) {
+        reinterpret_cast<double*>(B)[i] = D.ld(F_T, o);
+        reinterpret_cast<int32_t*>(B + 80 * N)[i] = (int32_t)(uint32_t)D.ld<uint64_t>(F_FLAGS, o);
+        put_u64(B + 84 * N + 8 * (uint64_t)i, D.ld<uint64_t>(F_NCOLL, o));
+        put_u64(B + 92 * N + 8 * (uint64_t)i, D.ld<uint64_t>(F_NBOUNCE, o));
+        put_u64(B + 100 * N + 8 * (uint64_t)i, D.ld<uint64_t>(F_NSTEPS, o));
+    }
+    const int fx[3] = {F_PX, F_VX, F_SX};
+    const uint64_t off[3] = {8 * N, 32 * N, 56 * N};
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        if (live) {
+            stage[3 * tid + 0] = D.ld(fx[f] + 0, o);
+            stage[3 * tid + 1] = D.ld(fx[f] + 1, o);
+            stage[3 * tid + 2] = D.ld(fx[f] + 2, o);
+        }
+        __syncthreads();
+        double* dst = reinterpret_cast<double*>(B + off[f]) + 3 * (uint64_t)base;
+        for (uint32_t k = tid; k < 3 * cnt; k += 256) dst[k] = stage[k];
+        __syncthreads();
+    }
 }
 
 }  // namespace st
@@ -436,6 +481,61 @@ static int save_ckpt(st_handle* h, const uint32_t* list, uint32_t n_list) {
     return ST_OK;
 }
 
+// The chunk loop as a captured HIP graph of K chunks (K even, so the list
+// buffers alternate back to list 0), replayed until the device reports an
+// empty next list: one host round trip per K chunks.  Kernel shapes are fixed
+// (k_kin over all n lanes, k_spin over its persistent grid, k_fixup over 64
+// groups); each reads this chunk's list length and failure count on the
+// device.  The graph holds the run's arguments, so it is captured per call.
+static int run_pipeline_graph(st_handle* h, PipeArgs A, SpinFn spin, int kbs) {
+    const int K = std::min(64, (h->ext.graph_chunks + 1) / 2 * 2);
+    hipLaunchKernelGGL(k_graph_start, dim3(1), dim3(1), 0, h->stream, h->d_small, h->n);
+    HIPCHK(h, hipGetLastError());
+    A.n_list = 0;
+    A.n_list_dev = h->d_small + 4;
+    hipGraph_t g = nullptr;
+    HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (int c = 0; c < K && e == hipSuccess; ++c) {
+        A.list = h->d_list[c & 1];
+        A.next_list = h->d_list[(c & 1) ^ 1];
+        hipLaunchKernelGGL(k_chunk_head, dim3(1), dim3(1), 0, h->stream, h->d_small);
+        if (h->batch)
+            hipLaunchKernelGGL(k_kin<true>, dim3((h->n + kbs - 1) / kbs), dim3(kbs), 0, h->stream, h->P, h->D, A);
+        else
+            hipLaunchKernelGGL(k_kin<false>, dim3((h->n + kbs - 1) / kbs), dim3(kbs), 0, h->stream, h->P, h->D, A);
+        e = hipGetLastError();
+        if (e == hipSuccess && spin) e = spin(h, dim3(h->grid_spin), A);
+        if (e == hipSuccess) {
+            if (h->batch)
+                hipLaunchKernelGGL(k_fixup<true>, dim3(64), dim3(64), 0, h->stream, h->P, h->D, A, FIXUP_FROM_DEVICE);
+            else
+                hipLaunchKernelGGL(k_fixup<false>, dim3(64), dim3(64), 0, h->stream, h->P, h->D, A, FIXUP_FROM_DEVICE);
+            e = hipGetLastError();
+        }
+    }
+    const hipError_t ec = hipStreamEndCapture(h->stream, &g);
+    if (e != hipSuccess || ec != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        return fail(h, ST_E_HIP, "chunk graph capture failed: %s",
+                    hipGetErrorString(e != hipSuccess ? e : ec));
+    }
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(h, ST_E_HIP, "chunk graph instantiation failed: %s", hipGetErrorString(e));
+    uint32_t left = 0;
+    for (;;) {
+        e = hipGraphLaunch(x, h->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(&left, h->d_small + 1, sizeof left, hipMemcpyDeviceToHost, h->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess || left == 0) break;
+    }
+    hipGraphExecDestroy(x);
+    if (e != hipSuccess) return fail(h, ST_E_HIP, "chunk graph replay failed: %s", hipGetErrorString(e));
+    return ST_OK;
+}
+
 static int run_pipeline(st_handle* h, RunArgs R) {
     int r = ensure_pipeline(h);
     if (r) return r;
@@ -468,6 +568,7 @@ static int run_pipeline(st_handle* h, RunArgs R) {
     uint32_t n_list = h->n;
     int cur = 0;
     const SpinFn spin = integ == INT_NONE ? nullptr : pick_spin(h->ext.variant, integ, h->batch, h->any_pulse);
+    if (h->ext.graph_chunks > 0 && !h->ckpt_on && !h->ktime_on) return run_pipeline_graph(h, A, spin, kbs);
     while (n_list > 0) {
         A.list = h->d_list[cur];
         A.n_list = n_list;
@@ -552,6 +653,10 @@ static int check_ext(const st_ext* ext, st_ext& x) {
         return fail(nullptr, ST_E_ARG, "unknown variant %d", x.variant);
     if (x.engine != ST_ENGINE_PIPELINE && x.engine != ST_ENGINE_FUSED)
         return fail(nullptr, ST_E_ARG, "unknown engine %d", x.engine);
+    if (x.graph_chunks < 0 || x.graph_chunks > 64)
+        return fail(nullptr, ST_E_ARG, "graph_chunks must be in 0..64 (got %d)", x.graph_chunks);
+    if (x.graph_chunks > 0 && x.engine != ST_ENGINE_PIPELINE)
+        return fail(nullptr, ST_E_UNSUPPORTED, "graph_chunks needs the pipeline engine");
     return ST_OK;
 }
 

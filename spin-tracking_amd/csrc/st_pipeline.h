@@ -55,7 +55,12 @@ struct PipeArgs {
     uint32_t fail_cap;
     int32_t integ;           // integratorType
     const Phys* phys;        // batched handles: physics per configuration (F_CFG indexes it)
+    // graph-captured chunks: the list length is read from here (set on the
+    // device by the chunk head) instead of n_list; nullptr otherwise
+    const uint32_t* n_list_dev;
 };
+
+__device__ __forceinline__ uint32_t list_len(const PipeArgs& A) { return A.n_list_dev ? *A.n_list_dev : A.n_list; }
 
 // The physics of particle pid: its configuration's entry for a batched
 // handle, the single entry for a plain one (read through the table either
@@ -243,7 +248,7 @@ template <bool BATCH>
 __global__ __launch_bounds__(256) void k_kin(const SimParams P, const DevState D, const PipeArgs A) {
     const uint32_t li = blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long cnt[2] = {0, 0};   // events, trips
-    if (li < A.n_list) {
+    if (li < list_len(A)) {
         KinLane L;
         kin_load(D, A.list[li], L);
         const Phys& F = lane_phys<BATCH>(P, D, A.phys, L.pid);
@@ -352,9 +357,10 @@ __device__ __forceinline__ void spin_body(const SimParams& P, const DevState& D,
     unsigned long long cnt[4] = {0, 0, 0, 0};   // accepted, rejected, segments, trips
     enum { S_DONE = 0, S_NEXT = 1, S_INT = 2 };
     int phase = S_DONE;
+    const uint32_t n_list = list_len(A);
     {
         const uint32_t q = acquire_index(R.queue);
-        if (q < A.n_list) {
+        if (q < n_list) {
             spin_load<BATCH>(D, A, A.list[q], L);
             phase = S_NEXT;
         }
@@ -369,7 +375,7 @@ __device__ __forceinline__ void spin_body(const SimParams& P, const DevState& D,
                     spin_store(D, L);
                     phase = S_DONE;
                     const uint32_t q = acquire_index(R.queue);
-                    if (q < A.n_list) {
+                    if (q < n_list) {
                         spin_load<BATCH>(D, A, A.list[q], L);
                         phase = S_NEXT;
                         continue;
@@ -473,11 +479,23 @@ __global__ __launch_bounds__(256, 3) void k_spin_o3(const SimParams P, const Dev
 // (the failing segment's event included), then apply the reference's stop:
 // failureState = status, t/pos/v/S NaN, and every later snapshot block shows
 // the stopped particle.
+// One failing particle per lane, grid-stride; n_fail == FIXUP_FROM_DEVICE
+// reads the count the spin kernel left in *A.fail_count (graph-captured
+// chunks, where the host does not see it).
+constexpr uint32_t FIXUP_FROM_DEVICE = 0xffffffffu;
+template <bool BATCH>
+__device__ __forceinline__ void fixup_one(const SimParams& P, const DevState& D, const PipeArgs& A, uint32_t f);
+
 template <bool BATCH>
 __global__ __launch_bounds__(64) void k_fixup(const SimParams P, const DevState D, const PipeArgs A,
                                               uint32_t n_fail) {
-    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= n_fail) return;
+    const uint32_t nf = n_fail == FIXUP_FROM_DEVICE ? min(*A.fail_count, A.fail_cap) : n_fail;
+    for (uint32_t f = blockIdx.x * blockDim.x + threadIdx.x; f < nf; f += gridDim.x * blockDim.x)
+        fixup_one<BATCH>(P, D, A, f);
+}
+
+template <bool BATCH>
+__device__ __forceinline__ void fixup_one(const SimParams& P, const DevState& D, const PipeArgs& A, uint32_t f) {
     const uint32_t pid = A.fail_list[3 * f + 0];
     const uint32_t jf = A.fail_list[3 * f + 1];
     const int32_t status = (int32_t)A.fail_list[3 * f + 2];

@@ -59,7 +59,7 @@ ENGINE_FUSED = 1
 class Ext(C.Structure):
     _fields_ = [("pulse_axis", C.c_int32), ("variant", C.c_int32), ("block_size", C.c_int32),
                 ("engine", C.c_int32), ("seg_buffer_mb", C.c_int32), ("chunk_records", C.c_int32),
-                ("reserved", C.c_int32 * 2)]
+                ("graph_chunks", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Rng(C.Structure):

@@ -56,19 +56,19 @@ class Particle:
 
     def __init__(self, opt: Options, pulse_axis: int = 2, variant: int = L.VARIANT_EXACT,
                  block_size: int = 0, engine: int = L.ENGINE_PIPELINE, seg_buffer_mb: int = 0,
-                 chunk_records: int = 0):
+                 chunk_records: int = 0, graph_chunks: int = 0):
         self.lib = L.load()
         self.opt = Options.from_buffer_copy(bytes(opt))
         self.n = int(opt.numParticles)
         self.ext = Ext(int(pulse_axis), int(variant), int(block_size), int(engine), int(seg_buffer_mb),
-                       int(chunk_records))
+                       int(chunk_records), int(graph_chunks))
         h = C.c_void_p()
         check(self.lib.st_create(C.byref(self.opt), C.byref(self.ext), C.byref(h)), None)
         self.h = h
 
     @classmethod
     def batch(cls, cfgs, pulse_axis: int = 2, variant: int = L.VARIANT_EXACT, block_size: int = 0,
-              seg_buffer_mb: int = 0, chunk_records: int = 0) -> "Particle":
+              seg_buffer_mb: int = 0, chunk_records: int = 0, graph_chunks: int = 0) -> "Particle":
         """Batched handle (st_create_batch): the configurations ``cfgs`` (each
         an Options block with its own numParticles) back to back in one
         ensemble.  ``layout()`` gives the particle offsets."""
@@ -80,7 +80,7 @@ class Particle:
         self.n = int(sum(int(c.numParticles) for c in cfgs))
         self.opt.numParticles = self.n
         self.ext = Ext(int(pulse_axis), int(variant), int(block_size), L.ENGINE_PIPELINE, int(seg_buffer_mb),
-                       int(chunk_records))
+                       int(chunk_records), int(graph_chunks))
         h = C.c_void_p()
         check(self.lib.st_create_batch(arr, len(cfgs), C.byref(self.ext), C.byref(h)), None)
         self.h = h

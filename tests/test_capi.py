@@ -48,6 +48,13 @@ def test_error_path_without_device_call():
     x.pulse_axis, x.variant, x.engine = 2, _lib.VARIANT_EXACT, _lib.ENGINE_FUSED
     assert lib.st_create(C.byref(o), C.byref(x), C.byref(h)) == _lib.ST_E_UNSUPPORTED
     assert b"pipeline engine" in lib.st_last_error(None)
+    # the graph-captured chunk loop: 0..64 chunks, pipeline engine only
+    o.integratorType = 0
+    x.engine, x.graph_chunks = _lib.ENGINE_PIPELINE, 65
+    assert lib.st_create(C.byref(o), C.byref(x), C.byref(h)) == _lib.ST_E_ARG
+    assert b"graph_chunks" in lib.st_last_error(None)
+    x.engine, x.graph_chunks = _lib.ENGINE_FUSED, 4
+    assert lib.st_create(C.byref(o), C.byref(x), C.byref(h)) == _lib.ST_E_UNSUPPORTED
     assert lib.st_advance_to(None, 1.0) == _lib.ST_E_ARG
 
 

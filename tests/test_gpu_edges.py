@@ -65,3 +65,25 @@ def test_zero_length_interval(oracle):
         E.advance_to(t, prev_t=prev)
         prev = t
     compare(oracle, g, gs, gc, E, tol=TOL)
+
+
+@pytest.mark.parametrize("n", [70001, 256, 255])
+def test_snapshot_block_device_ragged(n):
+    # k_snapshot stages the coordinate fields of each 256-particle group in
+    # LDS; a ragged last group, odd N (4-byte aligned counter arrays) and
+    # exact multiples all give the reference's block byte for byte
+    opt = configs.he3_thermal(n=n, tf=0.05, ioutInt=0.05)
+    print(f"[edges] snapshot block n={n}", flush=True)
+    with st.Particle(opt, variant=st.VARIANT_FAST) as p:
+        p.initParticles()
+        p.runSimulation(0.05)
+        bb = p.block_bytes()
+        assert bb == 108 * n
+        buf = st._lib.DeviceBuffer(bb + 64)
+        p.snapshot_block_device(buf.ptr)
+        raw = p.copy_to_host(buf.ptr, bb + 64)
+        buf.free()
+        s = p.snapshot()
+    b = np.frombuffer(raw[:bb].tobytes(), dtype=st.snapshot_block_dtype(n))[0]
+    for f in ("t", "pos", "v", "S", "failure", "n_coll", "n_bounce", "n_steps"):
+        assert b[f].tobytes() == s[f].tobytes(), f
