@@ -1,0 +1,93 @@
+"""Golden fixtures (tests/golden/golden.npz, written by
+tests/golden/make_golden.py).
+
+* The C oracle reproduces the RNG fixture (from a big-int restatement that
+  shares no code with it), its own trajectory snapshots and its single
+  integrator calls bit for bit: any drift of the checked oracle shows here.
+* The GPU reproduces the trajectory fixtures without a CPU run: integer
+  state (bounces, collisions, segments, failures) exactly, t/pos/v/S within
+  1e-9 -- for He-3 with gas collisions, at most 2 of 64 particles may leave
+  1e-9 through the libm ulp sensitivity measured in test_gpu_parity.py.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import spintrack as st
+from spintrack.options import from_bytes
+from conftest import ROOT, to_oracle_opt
+
+GOLDEN = os.path.join(ROOT, "tests", "golden", "golden.npz")
+FIELDS = ("t", "pos", "v", "S", "failure", "n_coll", "n_bounce", "n_steps")
+
+
+@pytest.fixture(scope="module")
+def golden():
+    return np.load(GOLDEN)   # allow_pickle stays False
+
+
+def case_options(g, name):
+    return from_bytes(g[f"{name}_options"].tobytes())
+
+
+def test_rng_fixture(oracle, golden):
+    g = golden
+    for k in range(len(g["rng_seeds"])):
+        s = oracle.rng_seeded(int(g["rng_seeds"][k]), int(g["rng_indices"][k]))
+        assert [int(s[w]) for w in "xyzw"] == [int(v) for v in g["rng_state"][k]]
+        assert np.array_equal(oracle.xoshiro(s, g["rng_outputs"].shape[1]).astype(np.uint64), g["rng_outputs"][k])
+
+
+@pytest.mark.parametrize("name", ["neutrons", "he3", "type1", "type2", "type3", "type4", "type5"])
+def test_oracle_reproduces_trajectories(oracle, golden, name):
+    opt = case_options(golden, name)
+    targets = golden[f"{name}_targets"]
+    E = oracle.Ensemble(to_oracle_opt(opt), opt.numParticles)
+    snaps = [E.snapshot()]
+    prev = opt.t0
+    for t in targets:
+        E.advance_to(float(t), prev_t=prev)
+        prev = float(t)
+        snaps.append(E.snapshot())
+    for f in FIELDS:
+        assert np.stack([s[f] for s in snaps]).tobytes() == golden[f"{name}_{f}"].tobytes(), (name, f)
+
+
+def test_oracle_reproduces_integrator_calls(oracle, golden):
+    o = np.frombuffer(golden["integrator_options"].tobytes(), dtype=oracle.OPTIONS_DTYPE).copy().reshape(())
+    for row in golden["integrator_calls"]:
+        integ = int(row[0])
+        t0, tf, y, p0, v0, h0 = row[1], row[2], row[3:6], row[6:9], row[9:12], row[12]
+        o["rtol"] = o["atol"] = {4: 1e-5, 5: 1e-5}.get(integ, 1e-11)
+        stt, yo, ho, na, nr = oracle.integrate(o, t0, tf, y, p0, v0, h0, integrator=integ)
+        assert (stt, na, nr) == (int(row[13]), int(row[18]), int(row[19])), integ
+        assert yo.tobytes() == row[14:17].tobytes() and ho == row[17], integ
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("variant", [st.VARIANT_EXACT, st.VARIANT_FAST])
+@pytest.mark.parametrize("name", ["neutrons", "he3", "type1", "type2", "type3", "type4", "type5"])
+def test_gpu_matches_golden(golden, name, variant):
+    opt = case_options(golden, name)
+    targets = golden[f"{name}_targets"]
+    print(f"[golden] {name} variant={variant}", flush=True)
+    with st.Particle(opt, variant=variant) as p:
+        p.initParticles()
+        snaps = [p.snapshot()]
+        for t in targets:
+            p.runSimulation(float(t))
+            snaps.append(p.snapshot())
+    for f in ("failure", "n_coll", "n_bounce", "n_steps"):
+        assert np.array_equal(np.stack([s[f] for s in snaps]), golden[f"{name}_{f}"]), (name, f)
+    worst = np.zeros(opt.numParticles)
+    for f in ("t", "pos", "v", "S"):
+        got = np.stack([s[f] for s in snaps])
+        want = golden[f"{name}_{f}"]
+        if f == "t":
+            d = np.abs(got - want)
+        else:
+            d = np.abs(st.xyz(got.reshape(-1)) - st.xyz(want.reshape(-1))).max(axis=1).reshape(got.shape)
+        worst = np.maximum(worst, d.max(axis=0))
+    allowed = 2 if name == "he3" else 0
+    assert (worst > 1e-9).sum() <= allowed, (name, worst.max())
