@@ -443,8 +443,19 @@ static RunArgs base_args(st_handle* h) {
 // ---- pipeline --------------------------------------------------------------
 static int ensure_pipeline(st_handle* h) {
     if (h->d_recs) return ST_OK;
-    const uint64_t budget = (uint64_t)(h->ext.seg_buffer_mb > 0 ? h->ext.seg_buffer_mb : 16384) << 20;
+    // default budget: 64 GiB, at most 40% of the device memory free now
+    uint64_t budget = (uint64_t)65536 << 20;
+    if (h->ext.seg_buffer_mb > 0) {
+        budget = (uint64_t)h->ext.seg_buffer_mb << 20;
+    } else {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 0)
+            budget = std::min<uint64_t>(budget, (uint64_t)(0.4 * (double)free_b));
+    }
     uint64_t C = budget / ((uint64_t)h->n * sizeof(SegRec));
+    // a budget-derived depth stops at 4096 records (seconds of motion per
+    // chunk for the densest workloads): small ensembles need no more
+    C = std::min<uint64_t>(C, 4096);
     if (h->ext.chunk_records > 0) C = (uint64_t)h->ext.chunk_records;
     C = std::min<uint64_t>(std::max<uint64_t>(C, 16), 1u << 20);
     h->rec_cap = (uint32_t)C;
@@ -846,6 +857,12 @@ ST_API int st_init_subset(st_handle* h, uint32_t first, uint32_t stride) {
                            o.tf, nullptr, nullptr, nullptr, 1u);
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipMemsetAsync(h->d_work, 0, N_WORK * sizeof(unsigned long long), h->stream));
+    // the pipeline's record buffer (up to tens of GB) is allocated here, with
+    // the ensemble, rather than inside the first advance
+    if (h->ext.engine == ST_ENGINE_PIPELINE) {
+        const int r = ensure_pipeline(h);
+        if (r) return r;
+    }
     h->initialized = true;
     h->last_target = o.t0;
     return ST_OK;

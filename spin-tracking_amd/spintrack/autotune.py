@@ -22,7 +22,7 @@ from . import _lib as L
 from .options import output_times
 from .particle import Particle
 
-DEFAULT_CHUNKS = (128, 256, 512)
+DEFAULT_CHUNKS = (256, 512, 1024)
 DEFAULT_BLOCKS = (128, 256)
 
 
