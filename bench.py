@@ -425,17 +425,27 @@ def bench_adjoint(args, variant):
                 fd = (loss(A.with_params(opt, tp)) - loss(A.with_params(opt, tm))) / (2 * d)
                 r[st._lib.GRAD_PARAMS[i]] = abs(grad[i] - fd) / abs(fd)
             rel[step] = r
-        out["fd_rel_err_step_1e-3"] = rel[1e-3]
         best = {k: min(r[k] for r in rel.values()) for k in rel[1e-3]}
+        # the accuracy check: the same physics and loss with fixed steps,
+        # where the forward map is smooth in theta
+        fixed = adjoint_fd_fixed_step(variant)
+        out["fd_check"] = {"case": fixed["workload"],
+                           "params_within_1e-6_best_step": fixed["params_within_1e-6_best_step"],
+                           "params_within_1e-6_or_loss_resolution": fixed["params_within_1e-6_or_resolution"],
+                           "params": fixed["params"]}
+        out["fd_fixed_step"] = fixed
+        # the adaptive 50 s run: its loss carries a ~1e-12 noise floor from
+        # accept/reject decisions that move with theta, so weakly coupled
+        # parameters resolve only to that floor (informational)
+        ad = {"note": "adaptive steps: differences limited by the controller's accept/reject noise",
+              "rel_err_step_1e-3": rel[1e-3],
+              "loss_change_step_1e-3": {st._lib.GRAD_PARAMS[i]: 2e-3 * abs(theta[i] * grad[i])
+                                        for i in ADJ_PARAMS},
+              "params_within_1e-6": sum(v < 1e-6 for v in best.values())}
         if args.adjoint_fd_ladder:
-            out["fd_rel_err_step_1e-5"] = rel[1e-5]
-            out["fd_rel_err_best"] = best
-        # loss change of the 1e-3 difference: parameters whose change is
-        # within ~1e3x the loss's noise floor (~1e-12 rad) cannot resolve 1e-6
-        out["fd_loss_change_step_1e-3"] = {st._lib.GRAD_PARAMS[i]: 2e-3 * abs(theta[i] * grad[i])
-                                           for i in ADJ_PARAMS}
-        out["fd_params_within_1e-6"] = sum(v < 1e-6 for v in best.values())
-        out["fd_fixed_step"] = adjoint_fd_fixed_step(variant)
+            ad["rel_err_step_1e-5"] = rel[1e-5]
+            ad["rel_err_best"] = best
+        out["fd_adaptive"] = ad
     return out
 
 
@@ -641,6 +651,7 @@ def main():
         "value": value,
         "batched_over_plain": sweep["batched_over_plain"] if sweep else None,
         "adjoint_over_forward": adj["adjoint_over_forward"] if adj else None,
+        "adjoint_fd_check": adj.get("fd_check") if adj else None,
         "unit": "particle-seconds/s",
         "n_gpus": 1,
         "steps": steps,
