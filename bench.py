@@ -224,6 +224,67 @@ def bench_rk45(args, variant):
             "spin_TFLOPs": tf, "spin_frac_fp64_peak": tf / FP64_PEAK_TFLOPS}
 
 
+# The reference's own published runs (notebook stdout of its GPU build on an
+# unrecorded NVIDIA GPU; end-to-end totals in ms, init and output included):
+# name -> (params beyond the shared block, total ms, citation)
+REFERENCE_RUNS = {
+    "magnus_neutrons_no_gravity": (
+        {"numParticles": 65536, "diffuse": "0", "gravity": "false", "E": "75.0e5, 0, 0", "tf": "2000.0",
+         "ioutInt": "20", "integratorType": "2"},
+        27310, "scripts/ComparingVCrossENeutrons.ipynb:285 (params :93-121)"),
+    "magnus_neutrons_gravity": (
+        {"numParticles": 65536, "diffuse": "0", "gravity": "true", "E": "75.0e5, 0, 0", "tf": "2000.0",
+         "ioutInt": "20", "integratorType": "2"},
+        293964, "scripts/ComparingVCrossENeutrons.ipynb:1024"),
+    "trajectory_1M_specular": (
+        {"numParticles": 1000000, "diffuse": "0", "gravity": "true", "tf": "500.0", "ioutInt": "10.0",
+         "integratorType": "-1"},
+        11871, "scripts/PhysicsSanityChecks/EnergyConservationCheck.ipynb:192 (params :35-71)"),
+    "trajectory_1M_diffuse": (
+        {"numParticles": 1000000, "diffuse": "1.0", "gravity": "true", "tf": "500.0", "ioutInt": "10.0",
+         "integratorType": "-1"},
+        15130, "scripts/PhysicsSanityChecks/EnergyConservationCheck.ipynb:140"),
+}
+REFERENCE_COMMON = {"dist": "C", "V": "5.0", "T": "0.4", "gas_coll": "false", "B0": "3.0e-6, 0, 0",
+                    "numPerGPUBlock": "256", "t0": "0.0", "a": "0", "w": "0", "h": "1e-5",
+                    "keepStepSize": "false", "fixedStepSize": "false", "rtol": "1.0e-12", "atol": "1.0e-12",
+                    "hmax": "1.0", "hmin": "1.0e-9", "seed": "0", "m": "1.674927e-27",
+                    "gamma": "-1.832471850e8"}
+
+
+def bench_reference_runs(args):
+    """The reference's published GPU runs, repeated with the same params.txt
+    through the shipped C++ driver (spintrack_main: parse, seed, advance per
+    interval, write every snapshot to a binary file), timed end to end like
+    the reference's own total.  The file goes to a temporary directory."""
+    import subprocess
+    import tempfile
+
+    exe = os.path.join(ROOT, "spin-tracking_amd", "spintrack_main")
+    out = {}
+    with tempfile.TemporaryDirectory() as d:
+        for name, (extra, ref_ms, cite) in REFERENCE_RUNS.items():
+            kv = dict(REFERENCE_COMMON, **{k: str(v) for k, v in extra.items()})
+            params = os.path.join(d, name + ".txt")
+            with open(params, "w") as f:
+                f.write("".join(f"{k}, {v}\n" for k, v in kv.items()))
+            t0 = time.perf_counter()
+            r = subprocess.run([exe, params, os.path.join(d, name + ".bin")], capture_output=True, text=True,
+                               timeout=900)
+            wall = time.perf_counter() - t0
+            if r.returncode != 0:
+                raise RuntimeError(f"{name}: driver failed: {r.stderr[-500:]}")
+            total_ms = float(r.stdout.strip().splitlines()[-1])
+            os.remove(os.path.join(d, name + ".bin"))
+            psec = int(kv["numParticles"]) * float(kv["tf"])
+            out[name] = {"total_ms": total_ms, "wall_s": wall, "value": psec / (total_ms / 1e3),
+                         "reference_total_ms": ref_ms, "reference_value": psec / (ref_ms / 1e3),
+                         "speedup": ref_ms / total_ms, "reference_source": cite}
+            print(f"[bench] reference run {name}: {out[name]}", file=sys.stderr, flush=True)
+    return {"note": "same params.txt as the reference's published runs (its GPU build on an unrecorded "
+                    "NVIDIA GPU); end-to-end totals, output file included", "runs": out}
+
+
 def bench_snapshot(args):
     """The snapshot stream of the per-interval path: k_snapshot turns the
     SoA particle state into one block of the reference's file layout (112 B
@@ -550,6 +611,8 @@ def main():
     ap.add_argument("--rk45-particles", type=int, default=131072,
                     help="He-3 atoms of the RK45 leg (the first of the seeded ensemble)")
     ap.add_argument("--no-integrators", action="store_true", help="skip the integratorType 2-5 leg")
+    ap.add_argument("--no-reference-runs", action="store_true",
+                    help="skip repeating the reference's published runs through the C++ driver")
     ap.add_argument("--integrator-particles", type=int, default=131072)
     ap.add_argument("--no-dressing", action="store_true", help="skip the spin-dressing leg")
     ap.add_argument("--dressing-per", type=int, default=262144, help="particles per species")
@@ -629,6 +692,10 @@ def main():
         rk45 = bench_rk45(args, variant)
         print(f"[bench] rk45: {rk45}", file=sys.stderr, flush=True)
 
+    refruns = None
+    if not args.no_reference_runs:
+        refruns = bench_reference_runs(args)
+
     snap = bench_snapshot(args)
     print(f"[bench] snapshot stream: {snap}", file=sys.stderr, flush=True)
 
@@ -679,6 +746,7 @@ def main():
         "rk45": rk45,
         "integrators": integ,
         "snapshot_stream": snap,
+        "reference_runs": refruns,
         "dressing": dressing,
         "adjoint": adj,
     }

@@ -220,6 +220,13 @@ ST_API int st_get_trace(st_handle* h, double* out, uint32_t capacity, uint32_t* 
  * process; st_memcpy_d2h copies on the handle's stream and waits. */
 ST_API int st_device_alloc(size_t bytes, void** ptr);
 ST_API int st_device_free(void* ptr);
+/* Page-locked host memory for staging snapshot blocks: st_memcpy_d2h into it
+ * runs at the link's full rate, and a host thread can write one block to the
+ * output file while the device computes the next interval.  (No counterpart
+ * in the reference, whose logger reads the particle arrays directly,
+ * src/particle.cpp:852-862.) */
+ST_API int st_host_alloc(size_t bytes, void** ptr);
+ST_API int st_host_free(void* ptr);
 ST_API int st_memcpy_d2h(st_handle* h, void* dst, const void* src, size_t bytes);
 /* Device-to-device copy queued on the handle's stream (asynchronous). */
 ST_API int st_memcpy_d2d(st_handle* h, void* dst, const void* src, size_t bytes);

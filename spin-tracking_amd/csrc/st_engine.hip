@@ -527,13 +527,13 @@ static int run_pipeline_graph(st_handle* h, PipeArgs A, SpinFn spin, int kbs) {
     }
     const hipError_t ec = hipStreamEndCapture(h->stream, &g);
     if (e != hipSuccess || ec != hipSuccess) {
-        if (g) hipGraphDestroy(g);
+        if (g) (void)hipGraphDestroy(g);
         return fail(h, ST_E_HIP, "chunk graph capture failed: %s",
                     hipGetErrorString(e != hipSuccess ? e : ec));
     }
     hipGraphExec_t x = nullptr;
     e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
+    (void)hipGraphDestroy(g);
     if (e != hipSuccess) return fail(h, ST_E_HIP, "chunk graph instantiation failed: %s", hipGetErrorString(e));
     uint32_t left = 0;
     for (;;) {
@@ -542,7 +542,7 @@ static int run_pipeline_graph(st_handle* h, PipeArgs A, SpinFn spin, int kbs) {
         if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
         if (e != hipSuccess || left == 0) break;
     }
-    hipGraphExecDestroy(x);
+    (void)hipGraphExecDestroy(x);
     if (e != hipSuccess) return fail(h, ST_E_HIP, "chunk graph replay failed: %s", hipGetErrorString(e));
     return ST_OK;
 }
@@ -1309,6 +1309,20 @@ ST_API int st_device_free(void* ptr) {
     if (!ptr) return ST_OK;
     hipError_t e = hipFree(ptr);
     if (e != hipSuccess) return fail(nullptr, ST_E_HIP, "hipFree failed: %s", hipGetErrorString(e));
+    return ST_OK;
+}
+
+ST_API int st_host_alloc(size_t bytes, void** ptr) {
+    if (!ptr) return fail(nullptr, ST_E_ARG, "null pointer");
+    hipError_t e = hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault);
+    if (e != hipSuccess) return fail(nullptr, ST_E_HIP, "hipHostMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    return ST_OK;
+}
+
+ST_API int st_host_free(void* ptr) {
+    if (!ptr) return ST_OK;
+    hipError_t e = hipHostFree(ptr);
+    if (e != hipSuccess) return fail(nullptr, ST_E_HIP, "hipHostFree failed: %s", hipGetErrorString(e));
     return ST_OK;
 }
 

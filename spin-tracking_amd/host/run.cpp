@@ -1,10 +1,13 @@
 // run.cpp -- BinaryLogger (src/logger.cpp:8-38) and mainAnalysis
 // (src/simulation.cpp:8-62) over the C ABI.
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
 #include <iostream>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
+#include <thread>
 
 #include "st_host.h"
 
@@ -46,6 +49,65 @@ struct DeviceBuf {
     ~DeviceBuf() {
         if (p) st_device_free(p);
     }
+};
+
+struct HostBuf {
+    void* p = nullptr;
+    ~HostBuf() {
+        if (p) st_host_free(p);
+    }
+};
+
+// Writes snapshot blocks to the logger on its own thread, one block in
+// flight: the device computes interval i+1 while block i goes to the file.
+class BlockWriter {
+public:
+    BlockWriter(Logger& log, size_t bytes) : log_(log), bytes_(bytes), th_([this] { loop(); }) {}
+    ~BlockWriter() {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        th_.join();
+    }
+    // hands over a block; waits until the previous one is written
+    void submit(const void* block) {
+        std::unique_lock<std::mutex> g(m_);
+        cv_.wait(g, [this] { return pending_ == nullptr; });
+        pending_ = block;
+        cv_.notify_all();
+    }
+    // waits until every submitted block is written
+    void drain() {
+        std::unique_lock<std::mutex> g(m_);
+        cv_.wait(g, [this] { return pending_ == nullptr; });
+    }
+
+private:
+    void loop() {
+        std::unique_lock<std::mutex> g(m_);
+        for (;;) {
+            cv_.wait(g, [this] { return pending_ != nullptr || stop_; });
+            if (pending_) {
+                const void* b = pending_;
+                g.unlock();
+                log_.writeBlock(b, bytes_);
+                g.lock();
+                pending_ = nullptr;
+                cv_.notify_all();
+            } else if (stop_) {
+                return;
+            }
+        }
+    }
+    Logger& log_;
+    size_t bytes_;
+    std::mutex m_;
+    std::condition_variable cv_;
+    const void* pending_ = nullptr;
+    bool stop_ = false;
+    std::thread th_;
 };
 
 int report(st_handle* h, int rc, const char* what) {
@@ -109,14 +171,31 @@ int main_analysis(st_options opt, const char* output, const RunConfig& cfg, std:
             << ms << std::endl;
         return ST_OK;
     }
-    for (size_t i = 0; i < times.size(); ++i) {
-        const auto t0 = clk::now();
-        if ((rc = st_advance_to(H.h, times[i]))) return report(H.h, rc, "st_advance_to");
-        if ((rc = output_data(H.h, opt, log))) return rc;
-        if (!log.ok()) return report(nullptr, ST_E_ARG, "writing the output file");
-        const auto ms = std::chrono::duration_cast<std::chrono::milliseconds>(clk::now() - t0).count();
-        out << "iter " << i << ", duration " << times[i] << ", " << ms << std::endl;
+    // the reference's loop (src/simulation.cpp:84-92): advance, then log the
+    // snapshot; each block is built on the device, copied into one of two
+    // page-locked buffers and written by the writer thread while the next
+    // interval runs.  The file's bytes are those of the synchronous loop.
+    const size_t bb = st_snapshot_block_bytes(H.h);
+    DeviceBuf dblock;
+    HostBuf hb[2];
+    if ((rc = st_device_alloc(bb, &dblock.p))) return report(nullptr, rc, "st_device_alloc");
+    for (auto& b : hb)
+        if ((rc = st_host_alloc(bb, &b.p))) return report(nullptr, rc, "st_host_alloc");
+    {
+        BlockWriter writer(log, bb);
+        for (size_t i = 0; i < times.size(); ++i) {
+            const auto t0 = clk::now();
+            if ((rc = st_advance_to(H.h, times[i]))) return report(H.h, rc, "st_advance_to");
+            if ((rc = st_snapshot_block_device(H.h, dblock.p))) return report(H.h, rc, "st_snapshot_block_device");
+            // the writer may still hold the other buffer, never this one
+            if ((rc = st_memcpy_d2h(H.h, hb[i & 1].p, dblock.p, bb))) return report(H.h, rc, "st_memcpy_d2h");
+            writer.submit(hb[i & 1].p);
+            const auto ms = std::chrono::duration_cast<std::chrono::milliseconds>(clk::now() - t0).count();
+            out << "iter " << i << ", duration " << times[i] << ", " << ms << std::endl;
+        }
+        writer.drain();
     }
+    if (!log.ok()) return report(nullptr, ST_E_ARG, "writing the output file");
     return ST_OK;
 }
 

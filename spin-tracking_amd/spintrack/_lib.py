@@ -108,6 +108,8 @@ SIGNATURES = {
     "st_get_trace": (C.c_int, [_vp, _vp, _u32, C.POINTER(_u32)]),
     "st_device_alloc": (C.c_int, [_sz, C.POINTER(_vp)]),
     "st_device_free": (C.c_int, [_vp]),
+    "st_host_alloc": (C.c_int, [_sz, C.POINTER(_vp)]),
+    "st_host_free": (C.c_int, [_vp]),
     "st_memcpy_d2h": (C.c_int, [_vp, _vp, _vp, _sz]),
     "st_memcpy_d2d": (C.c_int, [_vp, _vp, _vp, _sz]),
     "st_timer_start": (C.c_int, [_vp]),
