@@ -465,6 +465,7 @@ constexpr double CT1 = 1.0 / 150.0, CT2 = 0.0, CT3 = -3.0 / 100.0, CT4 = 16.0 / 
 // (:298), exactly as the reference does.
 struct RkState {
     double t, tf, lastH, prev_ratio;
+    double prev_lr;      // fast variant: ln(prev_ratio)
     d3 yy1;
     int32_t nstep;
     bool stop;
@@ -478,6 +479,7 @@ ST_HD __forceinline__ void rk_begin(RkState& R, const Seg& S, double t_end, doub
     R.lastH = h;
     R.yy1 = d3{0.0, 0.0, 0.0};
     R.prev_ratio = 1.0;
+    R.prev_lr = 0.0;
 }
 
 // One iteration of the reference's while(1) (:267-335).  Returns 0 to
@@ -556,7 +558,16 @@ ST_HD inline int rk_trial(const SimParams& P, const Phys& Fp, RkState& R, d3& y,
     error = std_max(error, 1.0E-16);
     const double tol = P.rtol;
     const double ratio = tol / error;
-    double q_ = pow(ratio, 0.7 / 7.0) * pow(R.prev_ratio, -0.4 / 7.0);
+    double q_;
+    if constexpr (VAR == VAR_FAST) {
+        // ratio^0.1 * prev^(-0.4/7) as one exp of a sum of logs, the previous
+        // trial's log carried over: one log and one exp instead of two pows
+        const double lr = log(ratio);
+        q_ = exp((0.7 / 7.0) * lr + (-0.4 / 7.0) * R.prev_lr);
+        R.prev_lr = lr;
+    } else {
+        q_ = pow(ratio, 0.7 / 7.0) * pow(R.prev_ratio, -0.4 / 7.0);
+    }
     q_ = std_min(q_, 4.0);
     if ((q_ > 0.81 && error < 2 * tol) || P.fixed) {
         nacc++;
@@ -709,13 +720,22 @@ ST_HD __forceinline__ int bounds_v2(const SimParams& P, RkState& R, double& h) {
     return 0;
 }
 
-// Controller and accept test (e.g. :400-424); 1 when the call ends.
+// Controller and accept test (e.g. :400-424); 1 when the call ends.  The
+// fast variant forms the step factor from logs, as in rk_trial.
+template <int VAR>
 ST_HD __forceinline__ int control_v2(const SimParams& P, RkState& R, d3& y, const d3& ynew, double& h,
                                      double error, uint32_t& nacc, uint32_t& nrej) {
     error = std_max(error, 1.0E-16);
     const double tol = P.rtol;
     const double ratio = tol / error;
-    double q_ = pow(ratio, 0.7 / 7.0) * pow(R.prev_ratio, -0.4 / 7.0);
+    double q_;
+    if constexpr (VAR == VAR_FAST) {
+        const double lr = log(ratio);
+        q_ = exp((0.7 / 7.0) * lr + (-0.4 / 7.0) * R.prev_lr);
+        R.prev_lr = lr;
+    } else {
+        q_ = pow(ratio, 0.7 / 7.0) * pow(R.prev_ratio, -0.4 / 7.0);
+    }
     q_ = std_min(q_, 4.0);
     if ((q_ > 0.81 && error < 2 * tol) || P.fixed) {
         nacc++;
@@ -789,7 +809,7 @@ ST_HD inline int rk2_trial(const SimParams& P, const Phys& Fp, RkState& R, d3& y
         ST_V3(te, h * (R1 * k1.x + R2 * k2.x + R3 * k3.x + R4 * k4.x + R5 * k5.x + R6 * k6.x),
               h * (R1 * k1.y + R2 * k2.y + R3 * k3.y + R4 * k4.y + R5 * k5.y + R6 * k6.y),
               h * (R1 * k1.z + R2 * k2.z + R3 * k3.z + R4 * k4.z + R5 * k5.z + R6 * k6.z));
-        return control_v2(P, R, y, ws, h, len3(te), nacc, nrej);
+        return control_v2<VAR>(P, R, y, ws, h, len3(te), nacc, nrej);
     } else if constexpr (INTEG == INT_RK45Q || INTEG == INT_RKF45Q) {
         // k_i = gamma*B(t + A_i h); the step rotates y by the product of
         // R(k_i, h W_i), k1 first; the comparison solution drops the sixth
@@ -807,7 +827,7 @@ ST_HD inline int rk2_trial(const SimParams& P, const Phys& Fp, RkState& R, d3& y
         const d3 ws = qv_mult(qw, y);
         const d3 te = qv_mult(qe, y);
         const d3 d{ws.x - te.x, ws.y - te.y, ws.z - te.z};
-        return control_v2(P, R, y, ws, h, len3(d), nacc, nrej);
+        return control_v2<VAR>(P, R, y, ws, h, len3(d), nacc, nrej);
     } else {
         static_assert(INTEG == INT_MAGNUS, "integrator type");
         using namespace cfet;
@@ -825,7 +845,7 @@ ST_HD inline int rk2_trial(const SimParams& P, const Phys& Fp, RkState& R, d3& y
             else y6 = out;
         }
         const d3 d{y8.x - y6.x, y8.y - y6.y, y8.z - y6.z};
-        return control_v2(P, R, y, y8, h, len3(d), nacc, nrej);
+        return control_v2<VAR>(P, R, y, y8, h, len3(d), nacc, nrej);
     }
 }
 
