@@ -82,7 +82,11 @@ typedef struct st_ext {
                                until every particle is done -- one host round trip per replay
                                instead of per chunk; 0 = host-driven chunk loop.  Runs with
                                checkpointing or per-kernel timing on use the host-driven loop. */
-    int32_t reserved;
+    int32_t spectrum_sum;   /* integratorType 6: which particles' covariance spectra each interval adds
+                               to the run's spectrum.  0 = the reference CPU path's: its in-place tree
+                               (src/particle.cpp:826-834) leaves particles 0 and 2^k < numParticles
+                               in slot 0, added in that order; 1 = every particle (aggregateSpectrum,
+                               src/particle.cpp:893-897), summed in a fixed order on the device */
 } st_ext;
 
 /* include/particle.h:52-59 */
@@ -285,6 +289,77 @@ ST_API int st_block_sums(st_handle* h, const void* blocks_dev, int32_t n_blocks,
  * the result. */
 #define ST_PHASE_STATS 6
 ST_API int st_phase_stats(st_handle* h, const void* blocks_dev, int32_t n_blocks, double* out, void* phi_dev);
+
+/* ---- Spectral path: integratorType 6 ------------------------------------
+ * The reference samples the noise field gamma*(v x E/c^2 + G.r + a sin(w t))/2
+ * along each trajectory segment at the multiples of the particle's h into
+ * ST_NW Goertzel filters (integrateSpectrum, src/integrator.cpp:730-743;
+ * SpectrumAggregator, src/floquet.cpp:111-147), sums the per-particle
+ * covariance spectra every interval (src/particle.cpp:824-834) and turns the
+ * total into a final Bloch vector with a Floquet-Markov master equation
+ * (floquet_integrate, src/integrator.cpp:869-889).  st_create accepts
+ * integratorType 6 on plain pipeline handles with w > 0; st_init computes
+ * the Floquet decomposition (floquet_diagonalize, :824-867) as the
+ * reference's initParticles does; every st_advance_to / st_run interval
+ * clears the aggregators, samples on the GPU and adds the interval's
+ * spectrum on the GPU.  The spin S is not integrated (it only turns NaN when
+ * the particle stops).  Work counter 0 and the per-particle "accepted"
+ * counter count samples. */
+#define ST_NW 17              /* include/floquet.h:34 */
+#define ST_NK 11              /* include/floquet.h:33 */
+#define ST_FLOQUET_NPROP 100  /* src/integrator.cpp:827 */
+
+/* floquetDiagonalization (include/floquet.h:99-106); quaternions (w, x, y, z) */
+typedef struct st_floquet {
+    double f_modes_0[4];
+    double f_energies[4];
+    double frequencies[ST_NW];
+    double dt;                 /* period / n_prop */
+    int32_t n_prop;
+    int32_t pad_;
+    double propagators[ST_FLOQUET_NPROP][4];
+} st_floquet;
+
+/* CovarianceSpectrum (include/floquet.h:45-62): the summed covariance
+ * matrices, variance[q][r][s] = (re, im), before normalisation. */
+typedef struct st_spectrum {
+    double frequencies[ST_NW];
+    double dt;
+    uint64_t n_samples;
+    double variance[ST_NW][3][3][2];
+} st_spectrum;
+
+/* == particle::cspec after the last interval (the CPU path's accumulated
+ * spectrum; the reference's GPU build never fills it). */
+ST_API int st_spectrum_get(st_handle* h, st_spectrum* out);
+/* == particle::getSpectrumAggregators (src/particle.cpp:899-901): the
+ * per-particle filter state of the last interval, 9*ST_NW doubles per
+ * particle (s1[ST_NW][3], s2[ST_NW][3], imag_diag[ST_NW][3]), and the
+ * per-particle sample counts.  Either pointer may be NULL. */
+ST_API int st_spectrum_aggregators(st_handle* h, double* agg, uint32_t* n_samples);
+/* The handle's Floquet decomposition (particle::fd). */
+ST_API int st_floquet_info(st_handle* h, st_floquet* out);
+/* == particle::floquetResults (src/particle.cpp:841-843; written by
+ * postProcess as "b_end", :845-850): the final Bloch vector. */
+ST_API int st_floquet_result(st_handle* h, double b_end[3]);
+
+/* Host-side Floquet routines (no GPU work), for callers that hold their own
+ * spectra.  == floquet_diagonalize(opt) (src/integrator.cpp:824-867). */
+ST_API int st_floquet_diagonalize(const st_options* opt, const st_ext* ext, st_floquet* out);
+/* == floquet_master_equation_rates (src/floquet.cpp:192-279).  power holds
+ * ST_NW (re, im) spectrum values at frequencies; Delta[2][2][ST_NK],
+ * X[2][2][ST_NK][2] and Gamma[2][2][ST_NK][2] are overwritten (any may be
+ * NULL); Zeta[2][2][2] and Omicron[2][2][2] are added to. */
+ST_API int st_floquet_rates(const double f_modes_0[4], const double f_energies[4], const double c_op[4],
+                            const double* propagators, int32_t n_prop, double period, const double* frequencies,
+                            const double* power, double* Delta, double* X, double* Gamma, double* Zeta,
+                            double* Omicron);
+/* == integrateFloquetMarkov (src/integrator.cpp:791-822), rho[2][2][2] in place. */
+ST_API int st_floquet_markov(double t0, double tf, double* rho, const double* Zeta, const double* Omicron);
+/* == floquet_integrate (src/integrator.cpp:869-889) of spec (normalised by
+ * spec->dt / spec->n_samples) with opt's yi, t0, tf and w. */
+ST_API int st_floquet_integrate(const st_floquet* fd, const st_spectrum* spec, const st_options* opt,
+                                double b_end[3]);
 
 /* Waits for all work queued on the handle's stream. */
 ST_API int st_synchronize(st_handle* h);

@@ -74,12 +74,26 @@ def default_options() -> np.ndarray:
     return o
 
 
+# SpectrumAggregator state of the C oracle (or_specagg)
+NW = 17
+SPECAGG_DTYPE = np.dtype({
+    "names": ["s1", "s2", "im", "n_samples"],
+    "formats": [("<f8", (NW, 3)), ("<f8", (NW, 3)), ("<f8", (NW, 3)), "<i4"],
+    "offsets": [0, NW * 24, 2 * NW * 24, 3 * NW * 24],
+    "itemsize": 3 * NW * 24 + 8,
+})
+
+
 class _Coords(C.Structure):
     _fields_ = [("x", C.c_double), ("y", C.c_double), ("z", C.c_double)]
 
 
 class _Ext(C.Structure):
     _fields_ = [("pulse_axis", C.c_int)]
+
+
+class _SpecCfg(C.Structure):
+    _fields_ = [("w", C.c_double * NW), ("dt", C.c_double)]
 
 
 class _Arrays(C.Structure):
@@ -151,6 +165,19 @@ def lib():
         L.or_init_particles.argtypes = [vp, vp, i32, u32, u32]
         L.or_run_simulation.restype = None
         L.or_run_simulation.argtypes = [vp, vp, vp, i32, d, vp, d, vp, i32]
+        L.or_run_simulation_spec.restype = None
+        L.or_run_simulation_spec.argtypes = [vp, vp, vp, i32, d, vp, d, vp, i32, vp, vp]
+        L.or_noise_term.restype = _Coords
+        L.or_noise_term.argtypes = [d, vp, d, _Coords, _Coords]
+        L.or_spectrum_update.restype = None
+        L.or_spectrum_update.argtypes = [vp, vp, _Coords]
+        L.or_integrate_spectrum.restype = i32
+        L.or_integrate_spectrum.argtypes = [d, d, vp, vp, _Coords, _Coords, vp, d]
+        L.or_covariances.restype = None
+        L.or_covariances.argtypes = [vp, i32, vp, vp]
+        L.or_integrate_hamiltonian.restype = i32
+        L.or_integrate_hamiltonian.argtypes = [d, d, vp, vp, vp, d]
+        L.or_sizeof_specagg.restype = i32
         L.or_sizeof_options.restype = i32
         L.or_sizeof_rng.restype = i32
         _lib = L
@@ -283,12 +310,70 @@ def integrate(opt, t0, tf, y, p_old, v_old, h, integrator=0, pulse_axis=2):
     return st, yy, hh[0], int(na[0]), int(nr[0])
 
 
+# ------------------------------------------------------- spectral path
+def spec_cfg(freqs, dt: float) -> _SpecCfg:
+    """SpectrumAggregator::initialize(freq, dt) (src/floquet.cpp:150-153)."""
+    c = _SpecCfg()
+    for i in range(NW):
+        c.w[i] = float(freqs[i])
+    c.dt = float(dt)
+    return c
+
+
+def noise_term(opt, t, t0, p_old, v_old):
+    """findNoiseTerm (src/integrator.cpp:72-80)."""
+    r = lib().or_noise_term(t, _ptr(opt), t0, _c3(p_old), _c3(v_old))
+    return np.array([r.x, r.y, r.z])
+
+
+def integrate_spectrum(opt, t0, tf, agg: np.ndarray, cfg: _SpecCfg, p_old, v_old, h) -> int:
+    """integrateSpectrum (src/integrator.cpp:730-743) into the 0-d aggregator
+    record agg; returns the number of samples."""
+    return lib().or_integrate_spectrum(t0, tf, _ptr(agg), C.addressof(cfg), _c3(p_old), _c3(v_old),
+                                       _ptr(opt), h)
+
+
+def covariances(agg: np.ndarray, cfg: _SpecCfg) -> np.ndarray:
+    """get_covariance_spectrum of every aggregator: (n, NW, 3, 3, 2) float64,
+    the last axis (re, im) (src/floquet.cpp:131-147)."""
+    agg = np.ascontiguousarray(agg)
+    out = np.zeros((agg.size, NW, 3, 3, 2))
+    lib().or_covariances(_ptr(agg), agg.size, C.addressof(cfg), _ptr(out))
+    return out
+
+
+def partial_tree_indices(n: int) -> list:
+    """The particles the reference's CPU path sums (src/particle.cpp:826-834):
+    its in-place tree over ipart = 0, 1, ... adds cspec_array[s] into
+    cspec_array[0] for s = 1, 2, 4, ... < n before any other slot is
+    reduced, so 0 and the powers of two below n make up the total."""
+    idx = [0]
+    s = 1
+    while s < n:
+        idx.append(s)
+        s *= 2
+    return idx
+
+
+def sum_spectra(cov: np.ndarray, n_samp: np.ndarray, mode: int):
+    """One interval's spectrum sum, in the reference's order: mode 0, the
+    CPU path's partial tree (((c0 + c1) + c2) + c4) ...; mode 1, every
+    particle in index order (aggregateSpectrum, src/particle.cpp:893-897)."""
+    idx = partial_tree_indices(len(cov)) if mode == 0 else list(range(len(cov)))
+    tot = cov[idx[0]].copy()
+    tn = int(n_samp[idx[0]])
+    for j in idx[1:]:
+        tot = tot + cov[j]
+        tn += int(n_samp[j])
+    return tot, tn
+
+
 # ---------------------------------------------------------------- ensemble
 class Ensemble:
     """The reference's particle arrays for n particles, on the host."""
 
     def __init__(self, opt: np.ndarray, n: int, first: int = 0, stride: int = 1,
-                 pulse_axis: int = 2):
+                 pulse_axis: int = 2, spectrum_sum: int = 0):
         self.opt = np.array(opt, dtype=OPTIONS_DTYPE).reshape(())
         self.n = int(n)
         self.arr = {k: np.zeros(self.n, dtype=dt) for k, dt in ARRAY_FIELDS.items()}
@@ -296,6 +381,18 @@ class Ensemble:
         self._a = _Arrays(**{k: _ptr(v) for k, v in self.arr.items()})
         self._ext = _ext(pulse_axis)
         lib().or_init_particles(_ptr(self.opt), C.addressof(self._a), self.n, first, stride)
+        self.spectral = int(self.opt["integratorType"]) == 6
+        if self.spectral:
+            # particle::initParticles -> initializeSpectra (include/particle.h:241-242,
+            # src/particle.cpp:887-891) and SpectrumAggregator::initialize (:520-521)
+            import floquet
+            self.fd = floquet.diagonalize(self.opt, pulse_axis)
+            self.cfg = spec_cfg(self.fd["frequencies"], self.fd["dt"])
+            self.agg = np.zeros(self.n, dtype=SPECAGG_DTYPE)
+            self.spectrum_sum = int(spectrum_sum)
+            self.cspec = np.zeros((NW, 3, 3, 2))   # CovarianceSpectrum::variance, (re, im)
+            self.cspec_n = 0
+            self.samples_total = np.zeros(self.n, dtype=np.uint64)   # per particle, all intervals
 
     def advance_to(self, next_t: float, stop_at=None, prev_t: float = 0.0,
                    nthreads: int = 0) -> None:
@@ -303,8 +400,23 @@ class Ensemble:
         if stop_at is not None:
             self._stop_at = np.ascontiguousarray(stop_at, dtype=np.float64)
             sp = _ptr(self._stop_at)
+        if self.spectral:
+            lib().or_run_simulation_spec(_ptr(self.opt), C.addressof(self._ext), C.addressof(self._a),
+                                         self.n, next_t, sp, prev_t, _ptr(self.n_ode), nthreads,
+                                         _ptr(self.agg), C.addressof(self.cfg))
+            self.samples_total += self.agg["n_samples"].astype(np.uint64)
+            tot, tn = sum_spectra(covariances(self.agg, self.cfg), self.agg["n_samples"], self.spectrum_sum)
+            self.cspec = self.cspec + tot
+            self.cspec_n += tn
+            return
         lib().or_run_simulation(_ptr(self.opt), C.addressof(self._ext), C.addressof(self._a),
                                 self.n, next_t, sp, prev_t, _ptr(self.n_ode), nthreads)
+
+    def floquet_result(self) -> np.ndarray:
+        """particle::floquetResults (src/particle.cpp:841-843): b_end."""
+        import floquet
+        var = self.cspec[..., 0] + 1j * self.cspec[..., 1]
+        return floquet.integrate(self.fd, var, self.cspec_n, self.fd["dt"], self.opt)
 
     def snapshot(self) -> dict:
         """The eight arrays BinaryLogger::writeSnapshot writes (src/logger.cpp:14-25)."""

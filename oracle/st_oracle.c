@@ -1096,6 +1096,139 @@ int or_integrate_magnus(double t0, double tf, or_coords *yio, or_coords p_old,
 #undef BLOCH
 
 /* ------------------------------------------------------------------ */
+/* Spectral path (integratorType 6)                                     */
+/* ------------------------------------------------------------------ */
+
+/* findNoiseTerm, src/integrator.cpp:72-80 (+ testNoise :36-38):
+ * gamma * ((1/c^2) v x E + G.r + a * sin(w t)), no pulse and no B0 */
+or_coords or_noise_term(double t, const or_options *opt, double t0, or_coords p_old,
+        or_coords v_old) {
+    or_coords p, v;
+    or_interpolate(t, t0, p_old, v_old, &p, &v, opt);
+    or_coords G;
+    G.x = dot3(p, opt->Gx);
+    G.y = dot3(p, opt->Gy);
+    G.z = dot3(p, opt->Gz);
+    or_coords N;
+    N.x = opt->noiseAmplitudes.x * sin(opt->noiseFrequencies.x * t);
+    N.y = opt->noiseAmplitudes.y * sin(opt->noiseFrequencies.y * t);
+    N.z = opt->noiseAmplitudes.z * sin(opt->noiseFrequencies.z * t);
+    const or_coords vxE = cross3(v, opt->E);
+    const double ic2 = 1.0 / C2;
+    or_coords w;
+    w.x = ((vxE.x * ic2 + G.x) + N.x) * opt->gamma;
+    w.y = ((vxE.y * ic2 + G.y) + N.y) * opt->gamma;
+    w.z = ((vxE.z * ic2 + G.z) + N.z) * opt->gamma;
+    return w;
+}
+
+/* SpectrumAggregator::update, src/floquet.cpp:111-118, with
+ * goertzel_stage_1 (:25-31), goertzel_stage_2_vector (:33-38) and
+ * covMat::add_outer(a, b, x, 0) (:105-108), which only touches imag_diag */
+void or_spectrum_update(or_specagg *a, const or_speccfg *c, or_coords x) {
+    for (int i = 0; i < OR_NW; i++) {
+        const double angle = c->w[i] * c->dt;
+        const double k = 2 * cos(angle);
+        or_coords ns;
+        ns.x = (x.x + k * a->s1[i].x) - a->s2[i].x;
+        ns.y = (x.y + k * a->s1[i].y) - a->s2[i].y;
+        ns.z = (x.z + k * a->s1[i].z) - a->s2[i].z;
+        a->s2[i] = a->s1[i];
+        a->s1[i] = ns;
+        const double cw = cos(angle), sw = sin(angle);
+        or_coords re, im;
+        re.x = a->s1[i].x - cw * a->s2[i].x;
+        re.y = a->s1[i].y - cw * a->s2[i].y;
+        re.z = a->s1[i].z - cw * a->s2[i].z;
+        im.x = sw * a->s2[i].x;
+        im.y = sw * a->s2[i].y;
+        im.z = sw * a->s2[i].z;
+        a->im[i].x = a->im[i].x + 2 * (im.x * x.x - re.x * 0.0);
+        a->im[i].y = a->im[i].y + 2 * (im.y * x.y - re.y * 0.0);
+        a->im[i].z = a->im[i].z + 2 * (im.z * x.z - re.z * 0.0);
+    }
+    a->n_samples += 1;
+}
+
+/* first_sample_point, src/integrator.cpp:724-728 */
+static double first_sample_point(double t0, double h) { return (floor(t0 / h) + 1) * h; }
+
+/* integrateSpectrum, src/integrator.cpp:730-743: the noise field at the
+ * multiples of h inside (t0, tf], halved, into the aggregator */
+int or_integrate_spectrum(double t0, double tf, or_specagg *agg, const or_speccfg *cfg,
+        or_coords p_old, or_coords v_old, const or_options *opt, double h) {
+    double t = first_sample_point(t0, h);
+    const double next_t = first_sample_point(tf, h);
+    int n_steps = 0;
+    while (t < next_t - (h / 2)) {
+        const or_coords B = or_noise_term(t, opt, t0, p_old, v_old);
+        const or_coords x = {B.x / 2, B.y / 2, B.z / 2};
+        or_spectrum_update(agg, cfg, x);
+        t += h;
+        n_steps++;
+    }
+    return n_steps;
+}
+
+/* SpectrumAggregator::get_covariance_spectrum, src/floquet.cpp:131-147
+ * (LAPLACE): per frequency the 3x3 matrix c c^H, c = a + i b from
+ * goertzel_stage_2, plus -i imag_diag on the diagonal; out[i][r][s] = (re, im) */
+void or_covariance(const or_specagg *a, const or_speccfg *cfg, double out[OR_NW][3][3][2]) {
+    for (int i = 0; i < OR_NW; i++) {
+        const double angle = cfg->w[i] * cfg->dt;
+        const double cw = cos(angle), sw = sin(angle);
+        const double s1[3] = {a->s1[i].x, a->s1[i].y, a->s1[i].z};
+        const double s2[3] = {a->s2[i].x, a->s2[i].y, a->s2[i].z};
+        const double id[3] = {a->im[i].x, a->im[i].y, a->im[i].z};
+        double re[3], im[3];
+        for (int r = 0; r < 3; r++) {
+            re[r] = s1[r] - cw * s2[r];
+            im[r] = sw * s2[r];
+        }
+        for (int r = 0; r < 3; r++)
+            for (int q = 0; q < 3; q++) {
+                /* (re_r + i im_r) * conj(re_q + i im_q) */
+                out[i][r][q][0] = re[r] * re[q] - im[r] * (-im[q]);
+                out[i][r][q][1] = re[r] * (-im[q]) + im[r] * re[q];
+            }
+        for (int r = 0; r < 3; r++) {
+            out[i][r][r][0] = out[i][r][r][0] + 0.0;
+            out[i][r][r][1] = out[i][r][r][1] + (-id[r]);
+        }
+    }
+}
+
+/* integrateHamiltonian, src/integrator.cpp:745-789: the order-8 CFET
+ * propagator (11 exponentials of the field at the Gauss-Legendre nodes) in
+ * fixed steps h (the last one shortened to end at tf), along the field at
+ * the origin at rest (p_old = v_old = 0); y <- q1 q2 ... q11 y */
+int or_integrate_hamiltonian(double t0, double tf, double *qio, const or_options *opt,
+        const or_ext *ext, double h) {
+    or_quat y = {qio[0], qio[1], qio[2], qio[3]};
+    const or_coords zero = {0.0, 0.0, 0.0};
+    double t = t0;
+    unsigned int nstep = 0;
+    int ret = 0;
+    while (t < tf) {
+        nstep++;
+        if (nstep > opt->nmax) { ret = -2; break; }
+        const double rest = tf - t;
+        if (h >= rest) h = rest;
+        or_coords B[5];
+        for (int i = 0; i < 5; i++) B[i] = or_cross_term(t + GL5X[i] * h, opt, ext, t0, zero, zero);
+        or_quat q[11];   /* q[k] = q_{k+1} */
+        for (int r = 0; r < 6; r++) q[r] = rodriguez_quat(cfet_exp(CF8[r], 0, B, 1.0), h);
+        for (int r = 4; r >= 0; r--) q[10 - r] = rodriguez_quat(cfet_exp(CF8[r], 1, B, 1.0), h);
+        or_quat P = q[0];
+        for (int k = 1; k < 11; k++) P = q_mult(P, q[k]);
+        y = q_mult(P, y);
+        t += h;
+    }
+    qio[0] = y.w; qio[1] = y.x; qio[2] = y.y; qio[3] = y.z;
+    return ret;
+}
+
+/* ------------------------------------------------------------------ */
 /* Ensemble: src/particle.cpp:638-837                                   */
 /* ------------------------------------------------------------------ */
 
@@ -1155,12 +1288,12 @@ void or_init_particles(const or_options *opt, const or_arrays *a, int n,
     }
 }
 
-/* src/particle.cpp:712-837 (integratorTypes 0-5; type 6, the spectral path,
- * is not restated, and types outside 0-6 leave the spin alone as the
- * reference's final else does). */
-void or_run_simulation(const or_options *opt, const or_ext *ext, const or_arrays *a,
+/* src/particle.cpp:712-837.  Types outside 0-6 leave the spin alone as the
+ * reference's final else does.  Type 6 samples the noise field into agg[j]
+ * (reset first, :746-747) and needs agg and cfg; it leaves S alone. */
+static void run_simulation(const or_options *opt, const or_ext *ext, const or_arrays *a,
         int n, double nextTOut, const double *stop_at, double prevTOut,
-        uint64_t *n_ode, int nthreads) {
+        uint64_t *n_ode, int nthreads, or_specagg *agg, const or_speccfg *cfg) {
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #pragma omp parallel for schedule(dynamic, 16)
@@ -1168,6 +1301,7 @@ void or_run_simulation(const or_options *opt, const or_ext *ext, const or_arrays
     (void)nthreads;
 #endif
     for (int j = 0; j < n; j++) {
+        if (agg) memset(agg + j, 0, sizeof *agg);
         double target = nextTOut;
         if (stop_at) {
             if (!(prevTOut < stop_at[j])) continue;
@@ -1189,6 +1323,8 @@ void or_run_simulation(const or_options *opt, const or_ext *ext, const or_arrays
         bool finished = false;
         int spinResult = 0;
         double tempH = h;
+        or_specagg sa;
+        if (agg) memset(&sa, 0, sizeof sa);
         while (!finished && !stop) {
             or_calc_next_collision_time(t, tf, v, pos, &next_gas, &dt, &coll_type,
                     &n_bounce, &n_coll, &finished, &wall_hit, &state, opt);
@@ -1214,6 +1350,9 @@ void or_run_simulation(const or_options *opt, const or_ext *ext, const or_arrays
                 else if (opt->integratorType == 5)
                     spinResult = or_integrate_rkf45q(t_old, t, &S, pos_old, v_old, opt, ext,
                             &tempH, &nacc, &nrej);
+                else if (opt->integratorType == 6 && agg)
+                    /* the CPU path drops the sample count (:777-778) */
+                    (void)or_integrate_spectrum(t_old, t, &sa, cfg, pos_old, v_old, opt, tempH);
                 else
                     spinResult = 0;
                 nseg++;
@@ -1249,6 +1388,7 @@ void or_run_simulation(const or_options *opt, const or_ext *ext, const or_arrays
         a->coll_type[j] = coll_type;
         a->wall_hit[j] = wall_hit;
         a->tf[j] = tf;
+        if (agg) agg[j] = sa;
         if (n_ode) {
             n_ode[3 * (size_t)j + 0] += nacc;
             n_ode[3 * (size_t)j + 1] += nrej;
@@ -1256,3 +1396,26 @@ void or_run_simulation(const or_options *opt, const or_ext *ext, const or_arrays
         }
     }
 }
+
+void or_run_simulation(const or_options *opt, const or_ext *ext, const or_arrays *a,
+        int n, double nextTOut, const double *stop_at, double prevTOut,
+        uint64_t *n_ode, int nthreads) {
+    run_simulation(opt, ext, a, n, nextTOut, stop_at, prevTOut, n_ode, nthreads, NULL, NULL);
+}
+
+void or_run_simulation_spec(const or_options *opt, const or_ext *ext, const or_arrays *a,
+        int n, double nextTOut, const double *stop_at, double prevTOut,
+        uint64_t *n_ode, int nthreads, or_specagg *agg, const or_speccfg *cfg) {
+    run_simulation(opt, ext, a, n, nextTOut, stop_at, prevTOut, n_ode, nthreads, agg, cfg);
+}
+
+/* get_covariance_spectrum of particles 0..n-1 into out[n][OR_NW][3][3][2] */
+void or_covariances(const or_specagg *agg, int n, const or_speccfg *cfg, double *out) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int j = 0; j < n; j++)
+        or_covariance(agg + j, cfg, (double (*)[3][3][2])(out + (size_t)j * OR_NW * 18));
+}
+
+int or_sizeof_specagg(void) { return (int)sizeof(or_specagg); }

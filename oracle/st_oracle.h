@@ -25,6 +25,12 @@
  *                                 findCrossTerm, Bloch
  *   src/integrator.cpp:90-249     integrateDOP (+ include/coeff.h:6-176)
  *   src/integrator.cpp:251-337    integrateRK45 (+ include/coeff.h:178-218)
+ *   src/integrator.cpp:339-723    types 2-5 (RKF45, quaternion forms, MagnusCFET)
+ *   src/integrator.cpp:72-80, 724-789, src/floquet.cpp:25-162
+ *                                 type 6: noise term, spectrum aggregator,
+ *                                 integrateHamiltonian (the Floquet host math,
+ *                                 src/floquet.cpp:188-301 and
+ *                                 src/integrator.cpp:791-889, is in floquet.py)
  */
 #ifndef ST_ORACLE_H
 #define ST_ORACLE_H
@@ -130,6 +136,33 @@ int or_integrate_rkf45q(double t0, double tf, or_coords *y, or_coords p_old,
         or_coords v_old, const or_options *opt, const or_ext *ext, double *h,
         uint64_t *n_acc, uint64_t *n_rej);
 
+/* ---- spectral path, integratorType 6 (include/floquet.h:33-97,
+ *      src/floquet.cpp:25-162, src/integrator.cpp:72-80, 724-789) ---- */
+#define OR_NW 17   /* include/floquet.h:33-34 */
+#define OR_NK 11
+/* SpectrumAggregator's state (include/floquet.h:75-97); of covMat only
+ * imag_diag ever changes (src/floquet.cpp:105-108) */
+typedef struct {
+    or_coords s1[OR_NW], s2[OR_NW], im[OR_NW];
+    int n_samples;
+} or_specagg;
+/* SpectrumAggregator::w and ::dt, set once by initialize (:150-153) */
+typedef struct {
+    double w[OR_NW];
+    double dt;
+} or_speccfg;
+or_coords or_noise_term(double t, const or_options *opt, double t0, or_coords p_old,
+        or_coords v_old);
+void or_spectrum_update(or_specagg *a, const or_speccfg *c, or_coords x);
+int or_integrate_spectrum(double t0, double tf, or_specagg *agg, const or_speccfg *cfg,
+        or_coords p_old, or_coords v_old, const or_options *opt, double h);
+void or_covariance(const or_specagg *a, const or_speccfg *cfg, double out[OR_NW][3][3][2]);
+void or_covariances(const or_specagg *agg, int n, const or_speccfg *cfg, double *out);
+/* q = (w, x, y, z) in place */
+int or_integrate_hamiltonian(double t0, double tf, double *q, const or_options *opt,
+        const or_ext *ext, double h);
+int or_sizeof_specagg(void);
+
 /* ---- ensemble (initParticlesCPU / runSimulationCPU) ----
  * first/stride select the global particle indices first + j*stride for the
  * local slot j, so a deterministic subset of a large run can be replayed. */
@@ -144,6 +177,12 @@ void or_init_particles(const or_options *opt, const or_arrays *a, int n,
 void or_run_simulation(const or_options *opt, const or_ext *ext, const or_arrays *a,
         int n, double nextTOut, const double *stop_at, double prevTOut,
         uint64_t *n_ode, int nthreads);
+
+/* The same with the type-6 aggregators agg[n] (reset at the start of the
+ * call) and their frequencies cfg. */
+void or_run_simulation_spec(const or_options *opt, const or_ext *ext, const or_arrays *a,
+        int n, double nextTOut, const double *stop_at, double prevTOut,
+        uint64_t *n_ode, int nthreads, or_specagg *agg, const or_speccfg *cfg);
 
 int or_sizeof_options(void);
 int or_sizeof_rng(void);

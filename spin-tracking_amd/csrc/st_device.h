@@ -33,7 +33,8 @@ constexpr double DBL_MAX_D = 1.79769313486231570815e+308; // DBL_MAX
 // arithmetic variants of the lane engine
 enum { VAR_EXACT = 0, VAR_FAST = 1 };
 // integrator types (options.integratorType; src/particle.cpp:568-592)
-enum { INT_NONE = -1, INT_DOP853 = 0, INT_RK45 = 1, INT_MAGNUS = 2, INT_RK45Q = 3, INT_RKF45 = 4, INT_RKF45Q = 5 };
+enum { INT_NONE = -1, INT_DOP853 = 0, INT_RK45 = 1, INT_MAGNUS = 2, INT_RK45Q = 3, INT_RKF45 = 4, INT_RKF45Q = 5,
+       INT_SPECTRUM = 6 };
 
 struct d3 {
     double x, y, z;

@@ -30,8 +30,10 @@
 #include "../../include/spintrack.h"
 #include "st_adjoint.h"
 #include "st_analysis.h"
+#include "st_floquet.h"
 #include "st_fused.h"
 #include "st_pipeline.h"
+#include "st_spectrum.h"
 
 static_assert(sizeof(st_options) == 440, "st_options must match the reference's 440-byte options");
 static_assert(offsetof(st_options, nmax) == 360, "st_options layout");
@@ -251,6 +253,14 @@ struct st_handle {
     char* d_adj = nullptr;          // A_COUNT fields
     char* d_replay = nullptr;       // kinematic state of the replayed chunk
     char* d_scratch = nullptr;      // next list, counters, failure list of replays
+    // spectral path (integratorType 6): per-particle aggregators, the run's
+    // summed covariance spectrum and the Floquet decomposition of the drive
+    SpecArgs spec;                  // aggregator pointers and filter constants
+    double* d_spec = nullptr;       // SPEC_ENTRIES doubles
+    unsigned long long* d_spec_n = nullptr;
+    double* d_spec_part = nullptr;  // SPEC_RED_BLOCKS * (SPEC_ENTRIES + 1)
+    floquet::Diag fd;
+    int grid_spec = 0;
     std::string err;
 };
 
@@ -556,8 +566,13 @@ static int run_pipeline(st_handle* h, RunArgs R) {
     }
     h->ckpt_run = R;
     const int integ = h->opt.integratorType;
-    if (integ != INT_NONE && (r = persistent_grid(h, spin_symbol(h->ext.variant, integ, h->batch, h->any_pulse), &h->grid_spin)))
+    const bool spectral = integ == INT_SPECTRUM;
+    if (spectral) {
+        if ((r = persistent_grid(h, (const void*)&k_spectrum<SPEC_FB>, &h->grid_spec))) return r;
+    } else if (integ != INT_NONE &&
+               (r = persistent_grid(h, spin_symbol(h->ext.variant, integ, h->batch, h->any_pulse), &h->grid_spin))) {
         return r;
+    }
     const int bs = 256;
     // lanes per workgroup of the kinematics kernel (a launch-shape knob)
     const int kbs = h->ext.block_size > 0 ? std::min(256, (h->ext.block_size + 63) / 64 * 64) : 256;
@@ -578,13 +593,16 @@ static int run_pipeline(st_handle* h, RunArgs R) {
     A.phys = h->d_phys;
     uint32_t n_list = h->n;
     int cur = 0;
-    const SpinFn spin = integ == INT_NONE ? nullptr : pick_spin(h->ext.variant, integ, h->batch, h->any_pulse);
-    if (h->ext.graph_chunks > 0 && !h->ckpt_on && !h->ktime_on) return run_pipeline_graph(h, A, spin, kbs);
+    const SpinFn spin =
+        (integ == INT_NONE || spectral) ? nullptr : pick_spin(h->ext.variant, integ, h->batch, h->any_pulse);
+    if (h->ext.graph_chunks > 0 && !h->ckpt_on && !h->ktime_on && !spectral)
+        return run_pipeline_graph(h, A, spin, kbs);
     while (n_list > 0) {
         A.list = h->d_list[cur];
         A.n_list = n_list;
         A.next_list = h->d_list[cur ^ 1];
-        HIPCHK(h, hipMemsetAsync(h->d_small, 0, 4 * sizeof(uint32_t), h->stream));
+        // [0] spin queue, [1] next count, [2] failures, [4 + b] queue of spectral block b
+        HIPCHK(h, hipMemsetAsync(h->d_small, 0, 8 * sizeof(uint32_t), h->stream));
         if (h->ktime_on) HIPCHK(h, hipEventRecord(h->kev[0], h->stream));
         if (h->batch)
             hipLaunchKernelGGL(k_kin<true>, dim3((n_list + kbs - 1) / kbs), dim3(kbs), 0, h->stream, h->P, h->D, A);
@@ -600,6 +618,19 @@ static int run_pipeline(st_handle* h, RunArgs R) {
             if (h->ktime_on) HIPCHK(h, hipEventRecord(h->kev[2], h->stream));
             HIPCHK(h, spin(h, dim3(grid), A));
             if (h->ktime_on) HIPCHK(h, hipEventRecord(h->kev[3], h->stream));
+        } else if (spectral) {
+            // one launch per register block of filters, each with its own queue
+            const int grid = std::min<int>(h->grid_spec, (int)((n_list + 255) / 256));
+            if (h->ktime_on) HIPCHK(h, hipEventRecord(h->kev[2], h->stream));
+            for (int b = 0; b < SPEC_NB; ++b) {
+                PipeArgs Ab = A;
+                Ab.R.queue = h->d_small + 4 + b;
+                SpecArgs S = h->spec;
+                S.block = b;
+                hipLaunchKernelGGL(k_spectrum<SPEC_FB>, dim3(grid), dim3(256), 0, h->stream, h->P, h->D, Ab, S);
+                HIPCHK(h, hipGetLastError());
+            }
+            if (h->ktime_on) HIPCHK(h, hipEventRecord(h->kev[3], h->stream));
         }
         uint32_t counts[4];
         HIPCHK(h, hipMemcpyAsync(counts, h->d_small, sizeof counts, hipMemcpyDeviceToHost, h->stream));
@@ -609,7 +640,7 @@ static int run_pipeline(st_handle* h, RunArgs R) {
             HIPCHK(h, hipEventElapsedTime(&ms, h->kev[0], h->kev[1]));
             h->kms[0] += ms;
             h->klaunch[0]++;
-            if (spin) {
+            if (spin || spectral) {
                 HIPCHK(h, hipEventElapsedTime(&ms, h->kev[2], h->kev[3]));
                 h->kms[1] += ms;
                 h->klaunch[1]++;
@@ -649,6 +680,93 @@ static int run_fused(st_handle* h, RunArgs R, bool persist) {
     return ST_OK;
 }
 
+// ---- spectral path (integratorType 6) -----------------------------------------
+static int spec_setup(st_handle* h) {
+    SpecArgs& S = h->spec;
+    memset(&S, 0, sizeof S);
+    S.astride = field_stride(h->n) / 8;
+    HIPCHK(h, hipMalloc(&S.agg, sizeof(double) * AGG_FIELDS * S.astride));
+    HIPCHK(h, hipMalloc(&S.nsamp, sizeof(uint32_t) * h->n));
+    HIPCHK(h, hipMalloc(&h->d_spec, sizeof(double) * SPEC_ENTRIES));
+    HIPCHK(h, hipMalloc(&h->d_spec_n, sizeof(unsigned long long)));
+    HIPCHK(h, hipMalloc(&h->d_spec_part, sizeof(double) * SPEC_RED_BLOCKS * (SPEC_ENTRIES + 1)));
+    const st_options& o = h->opt;
+    S.na[0] = o.noiseAmplitudes.x; S.na[1] = o.noiseAmplitudes.y; S.na[2] = o.noiseAmplitudes.z;
+    S.nw[0] = o.noiseFrequencies.x; S.nw[1] = o.noiseFrequencies.y; S.nw[2] = o.noiseFrequencies.z;
+    S.noise_on = (S.na[0] != 0.0 || S.na[1] != 0.0 || S.na[2] != 0.0) ? 1 : 0;
+    return ST_OK;
+}
+
+// particle::initParticles -> initializeSpectra (include/particle.h:241-242,
+// src/particle.cpp:887-891) and SpectrumAggregator::initialize
+// (:520-521, 707-708): the Floquet decomposition of the drive, the filter
+// constants of its NW frequencies, and empty aggregators and spectrum.
+static int spec_init(st_handle* h) {
+    floquet::diagonalize(h->P, h->opt.w, h->opt.nmax, h->fd);
+    for (int q = 0; q < NW; ++q) {
+        const double angle = h->fd.freq[q] * h->fd.dt;
+        h->spec.k2[q] = 2 * cos(angle);
+        h->spec.cw[q] = cos(angle);
+        h->spec.sw[q] = sin(angle);
+    }
+    hipLaunchKernelGGL(k_spec_clear, dim3((h->n + 255) / 256), dim3(256), 0, h->stream, h->spec, h->n);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipMemsetAsync(h->d_spec, 0, sizeof(double) * SPEC_ENTRIES, h->stream));
+    HIPCHK(h, hipMemsetAsync(h->d_spec_n, 0, sizeof(unsigned long long), h->stream));
+    return ST_OK;
+}
+
+// One interval of the spectral path: clear the aggregators
+// (src/particle.cpp:746-747), run the pipeline to the interval's end, and
+// add the interval's covariance spectra to the run's (:824-834).
+static int spec_interval(st_handle* h, const RunArgs& R) {
+    hipLaunchKernelGGL(k_spec_clear, dim3((h->n + 255) / 256), dim3(256), 0, h->stream, h->spec, h->n);
+    HIPCHK(h, hipGetLastError());
+    const int r = run_pipeline(h, R);
+    if (r) return r;
+    if (h->ext.spectrum_sum == 0) {
+        hipLaunchKernelGGL(k_spec_partial, dim3(1), dim3(64), 0, h->stream, h->spec, h->n, h->d_spec, h->d_spec_n);
+    } else {
+        const int blocks = std::min<int>(SPEC_RED_BLOCKS, (int)((h->n + 255) / 256));
+        hipLaunchKernelGGL(k_spec_all, dim3(blocks), dim3(256), 0, h->stream, h->spec, h->n, h->d_spec_part);
+        HIPCHK(h, hipGetLastError());
+        hipLaunchKernelGGL(k_spec_final, dim3(1), dim3(320), 0, h->stream, h->d_spec_part, blocks, h->d_spec,
+                           h->d_spec_n);
+    }
+    HIPCHK(h, hipGetLastError());
+    return ST_OK;
+}
+
+static void fill_floquet(const floquet::Diag& fd, st_floquet* out) {
+    memset(out, 0, sizeof *out);
+    const Quat m = fd.f_modes_0, e = fd.f_energies;
+    const double mm[4] = {m.w, m.x, m.y, m.z}, ee[4] = {e.w, e.x, e.y, e.z};
+    memcpy(out->f_modes_0, mm, sizeof mm);
+    memcpy(out->f_energies, ee, sizeof ee);
+    memcpy(out->frequencies, fd.freq, sizeof fd.freq);
+    out->dt = fd.dt;
+    out->n_prop = fd.n_prop;
+    for (int i = 0; i < fd.n_prop && i < ST_FLOQUET_NPROP; ++i) {
+        out->propagators[i][0] = fd.props[i].w;
+        out->propagators[i][1] = fd.props[i].x;
+        out->propagators[i][2] = fd.props[i].y;
+        out->propagators[i][3] = fd.props[i].z;
+    }
+}
+
+static floquet::Diag read_floquet(const st_floquet* f) {
+    floquet::Diag fd;
+    fd.f_modes_0 = Quat{f->f_modes_0[0], f->f_modes_0[1], f->f_modes_0[2], f->f_modes_0[3]};
+    fd.f_energies = Quat{f->f_energies[0], f->f_energies[1], f->f_energies[2], f->f_energies[3]};
+    memcpy(fd.freq, f->frequencies, sizeof fd.freq);
+    fd.dt = f->dt;
+    fd.n_prop = f->n_prop;
+    for (int i = 0; i < f->n_prop; ++i)
+        fd.props.push_back(
+            Quat{f->propagators[i][0], f->propagators[i][1], f->propagators[i][2], f->propagators[i][3]});
+    return fd;
+}
+
 extern "C" {
 
 ST_API const char* st_version(void) { return "spintrack gfx950 exact,fast dop853,rk45 pipeline,fused"; }
@@ -668,16 +786,27 @@ static int check_ext(const st_ext* ext, st_ext& x) {
         return fail(nullptr, ST_E_ARG, "graph_chunks must be in 0..64 (got %d)", x.graph_chunks);
     if (x.graph_chunks > 0 && x.engine != ST_ENGINE_PIPELINE)
         return fail(nullptr, ST_E_UNSUPPORTED, "graph_chunks needs the pipeline engine");
+    if (x.spectrum_sum != 0 && x.spectrum_sum != 1)
+        return fail(nullptr, ST_E_ARG, "spectrum_sum must be 0 or 1 (got %d)", x.spectrum_sum);
     return ST_OK;
 }
 
 static int check_options(const st_options& o) {
     if (o.numParticles <= 0) return fail(nullptr, ST_E_ARG, "numParticles must be > 0 (got %d)", o.numParticles);
-    if (o.integratorType > 5 || o.integratorType < -1)
+    if (o.integratorType > 6 || o.integratorType < -1)
         return fail(nullptr, ST_E_UNSUPPORTED,
                     "integratorType %d is not implemented (0 = DOP853, 1 = RK45, 2 = MagnusCFET, 3 = RK45 "
-                    "quaternion, 4 = RKF45, 5 = RKF45 quaternion, -1 = trajectory only)",
+                    "quaternion, 4 = RKF45, 5 = RKF45 quaternion, 6 = noise spectrum / Floquet, -1 = "
+                    "trajectory only)",
                     o.integratorType);
+    if (o.integratorType == INT_SPECTRUM && !(o.w > 0.0 && std::isfinite(o.w)))
+        return fail(nullptr, ST_E_ARG, "integratorType 6 needs a drive frequency w > 0 (got %g)", o.w);
+    // the sampling step h: at most 2^40 samples per particle over the run,
+    // which also keeps every step well above the rounding of the sample time
+    if (o.integratorType == INT_SPECTRUM &&
+        !(o.h > 0.0 && std::isfinite(o.h) && std::max(std::fabs(o.t0), std::fabs(o.tf)) / o.h <= 0x1p40))
+        return fail(nullptr, ST_E_ARG, "integratorType 6 samples every h: h must be > 0 and >= max(|t0|, |tf|) / 2^40 "
+                    "(got h = %g)", o.h);
     return ST_OK;
 }
 
@@ -741,6 +870,12 @@ ST_API int st_create(const st_options* opt, const st_ext* ext, st_handle** out) 
         *out = nullptr;
         return ST_E_HIP;
     }
+    if (opt->integratorType == INT_SPECTRUM && spec_setup(h) != ST_OK) {
+        g_create_error = h->err;
+        st_destroy(h);
+        *out = nullptr;
+        return ST_E_HIP;
+    }
     return ST_OK;
 }
 
@@ -771,6 +906,8 @@ ST_API int st_create_batch(const st_options* cfgs, int32_t n_cfg, const st_ext* 
     std::vector<uint32_t> off(n_cfg + 1, 0);
     std::vector<uint32_t> seeds(n_cfg);
     std::vector<Phys> phys(n_cfg);
+    if (cfgs[0].integratorType == INT_SPECTRUM)
+        return fail(nullptr, ST_E_UNSUPPORTED, "integratorType 6 runs on plain handles (one drive per run)");
     for (int c = 0; c < n_cfg; ++c) {
         if ((r = check_options(cfgs[c]))) return r;
         const char* what = "";
@@ -825,7 +962,8 @@ ST_API void st_destroy(st_handle* h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     void* bufs[] = {h->state, h->d_targets, h->d_stop_at, h->d_work, h->d_small, h->d_trace, h->d_trace_count,
                     h->d_recs, h->d_list[0], h->d_list[1], h->d_seg_count, h->d_backup, h->d_fail,
-                    h->d_phys, h->d_cfg_off, h->d_cfg_seed, h->d_adj, h->d_replay, h->d_scratch};
+                    h->d_phys, h->d_cfg_off, h->d_cfg_seed, h->d_adj, h->d_replay, h->d_scratch,
+                    h->spec.agg, h->spec.nsamp, h->d_spec, h->d_spec_n, h->d_spec_part};
     free_ckpts(h);
     for (void* b : bufs)
         if (b) (void)hipFree(b);
@@ -863,6 +1001,10 @@ ST_API int st_init_subset(st_handle* h, uint32_t first, uint32_t stride) {
         const int r = ensure_pipeline(h);
         if (r) return r;
     }
+    if (o.integratorType == INT_SPECTRUM) {
+        const int r = spec_init(h);
+        if (r) return r;
+    }
     h->initialized = true;
     h->last_target = o.t0;
     return ST_OK;
@@ -891,7 +1033,9 @@ ST_API int st_advance_to(st_handle* h, double next_t) {
     R.t_single = next_t;
     R.t_prev = h->last_target;
     R.n_targets = 1;
-    const int r = h->ext.engine == ST_ENGINE_FUSED ? run_fused(h, R, false) : run_pipeline(h, R);
+    const int r = h->opt.integratorType == INT_SPECTRUM ? spec_interval(h, R)
+                  : h->ext.engine == ST_ENGINE_FUSED  ? run_fused(h, R, false)
+                                                      : run_pipeline(h, R);
     if (r) return r;
     h->last_target = next_t;
     return ST_OK;
@@ -908,6 +1052,21 @@ ST_API int st_run(st_handle* h, int32_t n_targets, const double* targets, void* 
         h->d_targets = nullptr;
         HIPCHK(h, hipMalloc(&h->d_targets, sizeof(double) * n_targets));
         h->targets_cap = n_targets;
+    }
+    if (h->opt.integratorType == INT_SPECTRUM) {
+        // the interval sums need every particle at each output time: one
+        // pipeline pass per interval, snapshot block k written at target k
+        for (int k = 0; k < n_targets; ++k) {
+            RunArgs R = base_args(h);
+            R.t_single = targets[k];
+            R.t_prev = h->last_target;
+            R.n_targets = 1;
+            R.blocks = blocks_dev ? (char*)blocks_dev + (size_t)k * R.block_bytes : nullptr;
+            const int r = spec_interval(h, R);
+            if (r) return r;
+            h->last_target = targets[k];
+        }
+        return ST_OK;
     }
     HIPCHK(h, hipMemcpyAsync(h->d_targets, targets, sizeof(double) * n_targets, hipMemcpyHostToDevice, h->stream));
     RunArgs R = base_args(h);
@@ -1135,6 +1294,150 @@ ST_API int st_phase_stats(st_handle* h, const void* blocks_dev, int32_t n_blocks
     (void)hipFree(d_off);
     (void)hipFree(d_out);
     if (e != hipSuccess) return fail(h, ST_E_HIP, "phase statistics failed: %s", hipGetErrorString(e));
+    return ST_OK;
+}
+
+// ---- spectral path entry points ---------------------------------------------
+static int need_spectral(st_handle* h) {
+    if (!h) return fail(nullptr, ST_E_ARG, "null handle");
+    if (h->opt.integratorType != INT_SPECTRUM)
+        return fail(h, ST_E_STATE, "not a spectral run (integratorType %d)", h->opt.integratorType);
+    if (!h->initialized) return fail(h, ST_E_STATE, "call st_init first");
+    return ST_OK;
+}
+
+ST_API int st_spectrum_get(st_handle* h, st_spectrum* out) {
+    int r = need_spectral(h);
+    if (r) return r;
+    if (!out) return fail(h, ST_E_ARG, "null argument");
+    memset(out, 0, sizeof *out);
+    unsigned long long n = 0;
+    HIPCHK(h, hipMemcpyAsync(out->variance, h->d_spec, sizeof(double) * SPEC_ENTRIES, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipMemcpyAsync(&n, h->d_spec_n, sizeof n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    out->n_samples = n;
+    memcpy(out->frequencies, h->fd.freq, sizeof out->frequencies);
+    out->dt = h->fd.dt;
+    return ST_OK;
+}
+
+ST_API int st_spectrum_aggregators(st_handle* h, double* agg, uint32_t* n_samples) {
+    int r = need_spectral(h);
+    if (r) return r;
+    if (agg) {
+        std::vector<double> dev((size_t)AGG_FIELDS * h->spec.astride);
+        HIPCHK(h, hipMemcpyAsync(dev.data(), h->spec.agg, sizeof(double) * dev.size(), hipMemcpyDeviceToHost,
+                                 h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        for (uint32_t i = 0; i < h->n; ++i)
+            for (int f = 0; f < AGG_FIELDS; ++f) agg[(size_t)i * AGG_FIELDS + f] = dev[(size_t)f * h->spec.astride + i];
+    }
+    if (n_samples) {
+        HIPCHK(h, hipMemcpyAsync(n_samples, h->spec.nsamp, sizeof(uint32_t) * h->n, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+    }
+    return ST_OK;
+}
+
+ST_API int st_floquet_info(st_handle* h, st_floquet* out) {
+    int r = need_spectral(h);
+    if (r) return r;
+    if (!out) return fail(h, ST_E_ARG, "null argument");
+    fill_floquet(h->fd, out);
+    return ST_OK;
+}
+
+ST_API int st_floquet_result(st_handle* h, double b_end[3]) {
+    int r = need_spectral(h);
+    if (r) return r;
+    if (!b_end) return fail(h, ST_E_ARG, "null argument");
+    st_spectrum sp;
+    if ((r = st_spectrum_get(h, &sp))) return r;
+    const st_options& o = h->opt;
+    const d3 b = floquet::integrate(h->fd, &sp.variance[0][0][0][0], (double)sp.n_samples, h->fd.dt,
+                                    d3{o.yi.x, o.yi.y, o.yi.z}, o.t0, o.tf, o.w);
+    b_end[0] = b.x; b_end[1] = b.y; b_end[2] = b.z;
+    return ST_OK;
+}
+
+ST_API int st_floquet_diagonalize(const st_options* opt, const st_ext* ext, st_floquet* out) {
+    if (!opt || !out) return fail(nullptr, ST_E_ARG, "st_floquet_diagonalize: null argument");
+    st_ext x;
+    int r = check_ext(ext, x);
+    if (r) return r;
+    if (!(opt->w > 0.0 && std::isfinite(opt->w))) return fail(nullptr, ST_E_ARG, "drive frequency w must be > 0");
+    floquet::Diag fd;
+    floquet::diagonalize(derive_params(*opt, x), opt->w, opt->nmax, fd);
+    fill_floquet(fd, out);
+    return ST_OK;
+}
+
+ST_API int st_floquet_rates(const double f_modes_0[4], const double f_energies[4], const double c_op[4],
+                            const double* propagators, int32_t n_prop, double period, const double* frequencies,
+                            const double* power, double* Delta, double* X, double* Gamma, double* Zeta,
+                            double* Omicron) {
+    if (!f_modes_0 || !f_energies || !c_op || !propagators || n_prop <= 0 || !frequencies || !power || !Zeta ||
+        !Omicron)
+        return fail(nullptr, ST_E_ARG, "st_floquet_rates: null argument or n_prop <= 0");
+    using floquet::cd;
+    std::vector<Quat> props(n_prop);
+    for (int i = 0; i < n_prop; ++i)
+        props[i] = Quat{propagators[4 * i], propagators[4 * i + 1], propagators[4 * i + 2], propagators[4 * i + 3]};
+    cd pw[NW];
+    for (int j = 0; j < NW; ++j) pw[j] = cd(power[2 * j], power[2 * j + 1]);
+    double D[2][2][NK];
+    cd Xc[2][2][NK], G[2][2][NK], Z[2][2], O[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) {
+            Z[i][j] = cd(Zeta[4 * i + 2 * j], Zeta[4 * i + 2 * j + 1]);
+            O[i][j] = cd(Omicron[4 * i + 2 * j], Omicron[4 * i + 2 * j + 1]);
+        }
+    floquet::rates(Quat{f_modes_0[0], f_modes_0[1], f_modes_0[2], f_modes_0[3]},
+                   Quat{f_energies[0], f_energies[1], f_energies[2], f_energies[3]},
+                   Quat{c_op[0], c_op[1], c_op[2], c_op[3]}, props.data(), n_prop, period, frequencies, pw, D, Xc, G,
+                   Z, O);
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) {
+            Zeta[4 * i + 2 * j] = Z[i][j].real(); Zeta[4 * i + 2 * j + 1] = Z[i][j].imag();
+            Omicron[4 * i + 2 * j] = O[i][j].real(); Omicron[4 * i + 2 * j + 1] = O[i][j].imag();
+            for (int k = 0; k < NK; ++k) {
+                const int e = (2 * i + j) * NK + k;
+                if (Delta) Delta[e] = D[i][j][k];
+                if (X) { X[2 * e] = Xc[i][j][k].real(); X[2 * e + 1] = Xc[i][j][k].imag(); }
+                if (Gamma) { Gamma[2 * e] = G[i][j][k].real(); Gamma[2 * e + 1] = G[i][j][k].imag(); }
+            }
+        }
+    return ST_OK;
+}
+
+ST_API int st_floquet_markov(double t0, double tf, double* rho, const double* Zeta, const double* Omicron) {
+    if (!rho || !Zeta || !Omicron) return fail(nullptr, ST_E_ARG, "st_floquet_markov: null argument");
+    using floquet::cd;
+    floquet::M2 m;
+    cd Z[2][2], O[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) {
+            m.a[i][j] = cd(rho[4 * i + 2 * j], rho[4 * i + 2 * j + 1]);
+            Z[i][j] = cd(Zeta[4 * i + 2 * j], Zeta[4 * i + 2 * j + 1]);
+            O[i][j] = cd(Omicron[4 * i + 2 * j], Omicron[4 * i + 2 * j + 1]);
+        }
+    floquet::markov(t0, tf, m, Z, O);
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) {
+            rho[4 * i + 2 * j] = m.a[i][j].real();
+            rho[4 * i + 2 * j + 1] = m.a[i][j].imag();
+        }
+    return ST_OK;
+}
+
+ST_API int st_floquet_integrate(const st_floquet* fd, const st_spectrum* spec, const st_options* opt,
+                                double b_end[3]) {
+    if (!fd || !spec || !opt || !b_end) return fail(nullptr, ST_E_ARG, "st_floquet_integrate: null argument");
+    if (fd->n_prop <= 0 || fd->n_prop > ST_FLOQUET_NPROP) return fail(nullptr, ST_E_ARG, "bad n_prop %d", fd->n_prop);
+    const floquet::Diag d = read_floquet(fd);
+    const d3 b = floquet::integrate(d, &spec->variance[0][0][0][0], (double)spec->n_samples, spec->dt,
+                                    d3{opt->yi.x, opt->yi.y, opt->yi.z}, opt->t0, opt->tf, opt->w);
+    b_end[0] = b.x; b_end[1] = b.y; b_end[2] = b.z;
     return ST_OK;
 }
 

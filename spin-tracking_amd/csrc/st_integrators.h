@@ -859,6 +859,11 @@ ST_HD __forceinline__ int rk_any_trial(const SimParams& P, const Phys& Fp, RkSta
         return rk2_trial<VAR, INTEG>(P, Fp, R, y, h, S, q, nacc, nrej);
 }
 
+// the eleven order-8 rows in the order their exponentials act (row 1
+// reversed first), for the host-side propagator integration
+constexpr double CFET8_ACTING[11][5] = {{ST_G1R}, {ST_G2R}, {ST_G3R}, {ST_G4R}, {ST_G5R}, {ST_G6},
+                                       {ST_G5},  {ST_G4},  {ST_G3},  {ST_G2},  {ST_G1}};
+
 #undef ST_G1
 #undef ST_G1R
 #undef ST_G2

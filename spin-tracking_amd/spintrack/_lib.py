@@ -59,7 +59,7 @@ ENGINE_FUSED = 1
 class Ext(C.Structure):
     _fields_ = [("pulse_axis", C.c_int32), ("variant", C.c_int32), ("block_size", C.c_int32),
                 ("engine", C.c_int32), ("seg_buffer_mb", C.c_int32), ("chunk_records", C.c_int32),
-                ("graph_chunks", C.c_int32), ("reserved", C.c_int32)]
+                ("graph_chunks", C.c_int32), ("spectrum_sum", C.c_int32)]
 
 
 class Rng(C.Structure):
@@ -79,6 +79,25 @@ class ParticleState(C.Structure):
                 ("n_steps", C.c_uint64), ("failure", C.c_int32), ("part_id", C.c_uint32),
                 ("stop", C.c_uint8), ("coll_type", C.c_uint8), ("wall_hit", C.c_uint8),
                 ("pad_", C.c_uint8 * 5)]
+
+
+NW = 17              # include/floquet.h:34
+NK = 11              # include/floquet.h:33
+FLOQUET_NPROP = 100
+AGG_FIELDS = 9 * NW  # s1[NW][3], s2[NW][3], imag_diag[NW][3] per particle
+
+
+class Floquet(C.Structure):
+    """floquetDiagonalization (include/floquet.h:99-106)."""
+    _fields_ = [("f_modes_0", C.c_double * 4), ("f_energies", C.c_double * 4),
+                ("frequencies", C.c_double * NW), ("dt", C.c_double), ("n_prop", C.c_int32),
+                ("pad_", C.c_int32), ("propagators", (C.c_double * 4) * FLOQUET_NPROP)]
+
+
+class Spectrum(C.Structure):
+    """CovarianceSpectrum (include/floquet.h:45-62), un-normalised."""
+    _fields_ = [("frequencies", C.c_double * NW), ("dt", C.c_double), ("n_samples", C.c_uint64),
+                ("variance", C.c_double * (NW * 18))]
 
 
 # every symbol include/spintrack.h declares, with its ctypes signature
@@ -120,6 +139,14 @@ SIGNATURES = {
     "st_adjoint": (C.c_int, [_vp, _vp, _vp, _vp]),
     "st_block_sums": (C.c_int, [_vp, _vp, _i32, _vp]),
     "st_phase_stats": (C.c_int, [_vp, _vp, _i32, _vp, _vp]),
+    "st_spectrum_get": (C.c_int, [_vp, C.POINTER(Spectrum)]),
+    "st_spectrum_aggregators": (C.c_int, [_vp, _vp, _vp]),
+    "st_floquet_info": (C.c_int, [_vp, C.POINTER(Floquet)]),
+    "st_floquet_result": (C.c_int, [_vp, _vp]),
+    "st_floquet_diagonalize": (C.c_int, [C.POINTER(Options), C.POINTER(Ext), C.POINTER(Floquet)]),
+    "st_floquet_rates": (C.c_int, [_vp, _vp, _vp, _vp, _i32, _d, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "st_floquet_markov": (C.c_int, [_d, _d, _vp, _vp, _vp]),
+    "st_floquet_integrate": (C.c_int, [C.POINTER(Floquet), C.POINTER(Spectrum), C.POINTER(Options), _vp]),
 }
 
 # parameter order of st_adjoint's gradient (15 per configuration)

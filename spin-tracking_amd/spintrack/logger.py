@@ -5,7 +5,10 @@ src/logger.cpp:27-29), then one block per snapshot holding, for all N
 particles, t[N] f64, pos[N] 3xf64, v[N] 3xf64, S[N] 3xf64, failureState[N]
 i32, n_coll[N] u64, n_bounce[N] u64, n_steps[N] u64 (writeSnapshot,
 src/logger.cpp:14-25) -- 108*N bytes per block.  There are
-floor((tf-t0)/ioutInt)+1 blocks in a complete run.
+floor((tf-t0)/ioutInt)+1 blocks in a complete run; a spectral run
+(integratorType 6) appends its final Bloch vector b_end as 3 f64
+(BinaryLogger::writeSingle, src/logger.cpp:31-33; particle::postProcess,
+src/particle.cpp:845-850).
 """
 from __future__ import annotations
 
@@ -43,6 +46,10 @@ class BinaryLogger:
         assert len(b) == 108 * self.n
         self.f.write(b)
 
+    def writeSingle(self, name: str, value) -> None:
+        """src/logger.cpp:31-33: the raw coords, no name."""
+        self.f.write(np.asarray(value, dtype="<f8").reshape(3).tobytes())
+
     def close(self) -> None:
         if self.f:
             self.f.close()
@@ -58,7 +65,8 @@ class BinaryLogger:
 def read_output(path: str):
     """Reads a binary output file: returns (options, dict of arrays shaped
     (n_snapshots, N)).  The counterpart of scripts/nedmSim.py:6-107 for the
-    current options layout."""
+    current options layout; a trailing b_end (spectral runs) is returned
+    under "b_end"."""
     with open(path, "rb") as f:
         raw = f.read()
     opt = from_bytes(raw[:440])
@@ -66,4 +74,8 @@ def read_output(path: str):
     bsz = 108 * n
     nblk = (len(raw) - 440) // bsz
     blocks = np.frombuffer(raw, dtype=snapshot_block_dtype(n), count=nblk, offset=440)
-    return opt, {k: np.array(blocks[k]) for k in SNAPSHOT_FIELDS}
+    out = {k: np.array(blocks[k]) for k in SNAPSHOT_FIELDS}
+    tail = len(raw) - 440 - nblk * bsz
+    if tail == 24:
+        out["b_end"] = np.frombuffer(raw, dtype="<f8", count=3, offset=440 + nblk * bsz).copy()
+    return opt, out

@@ -56,12 +56,12 @@ class Particle:
 
     def __init__(self, opt: Options, pulse_axis: int = 2, variant: int = L.VARIANT_EXACT,
                  block_size: int = 0, engine: int = L.ENGINE_PIPELINE, seg_buffer_mb: int = 0,
-                 chunk_records: int = 0, graph_chunks: int = 0):
+                 chunk_records: int = 0, graph_chunks: int = 0, spectrum_sum: int = 0):
         self.lib = L.load()
         self.opt = Options.from_buffer_copy(bytes(opt))
         self.n = int(opt.numParticles)
         self.ext = Ext(int(pulse_axis), int(variant), int(block_size), int(engine), int(seg_buffer_mb),
-                       int(chunk_records), int(graph_chunks))
+                       int(chunk_records), int(graph_chunks), int(spectrum_sum))
         h = C.c_void_p()
         check(self.lib.st_create(C.byref(self.opt), C.byref(self.ext), C.byref(h)), None)
         self.h = h
@@ -254,6 +254,43 @@ class Particle:
 
     def synchronize(self) -> None:
         check(self.lib.st_synchronize(self.h), self.h)
+
+    # ---- spectral path (integratorType 6) ----
+    def spectrum(self) -> dict:
+        """particle::cspec: the summed, un-normalised covariance spectrum."""
+        sp = L.Spectrum()
+        check(self.lib.st_spectrum_get(self.h, C.byref(sp)), self.h)
+        v = np.ctypeslib.as_array(sp.variance).reshape(L.NW, 3, 3, 2)
+        return dict(frequencies=np.array(sp.frequencies[:]), dt=sp.dt, n_samples=int(sp.n_samples),
+                    variance=v[..., 0] + 1j * v[..., 1], raw=v.copy())
+
+    def spectrumAggregators(self) -> dict:
+        """particle::getSpectrumAggregators: the last interval's filter state
+        s1, s2, imag_diag (n, NW, 3) and sample counts."""
+        agg = np.zeros((self.n, L.AGG_FIELDS))
+        ns = np.zeros(self.n, dtype=np.uint32)
+        check(self.lib.st_spectrum_aggregators(self.h, _ptr(agg), _ptr(ns)), self.h)
+        a = agg.reshape(self.n, 3, L.NW, 3)
+        return dict(s1=a[:, 0].copy(), s2=a[:, 1].copy(), im=a[:, 2].copy(), n_samples=ns)
+
+    def floquetInfo(self) -> dict:
+        """particle::fd (floquet_diagonalize of the run's drive)."""
+        f = L.Floquet()
+        check(self.lib.st_floquet_info(self.h, C.byref(f)), self.h)
+        from .floquet import floquet_dict
+        return floquet_dict(f)
+
+    def floquetResults(self) -> np.ndarray:
+        """particle::floquetResults (src/particle.cpp:841-843): b_end."""
+        b = np.zeros(3)
+        check(self.lib.st_floquet_result(self.h, _ptr(b)), self.h)
+        return b
+
+    def postProcess(self, logger) -> None:
+        """particle::postProcess (src/particle.cpp:845-850): a spectral run
+        appends b_end to the output."""
+        if int(self.opt.integratorType) == 6:
+            logger.writeSingle("b_end", self.floquetResults())
 
 
 def xyz(a: np.ndarray) -> np.ndarray:

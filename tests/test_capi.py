@@ -37,10 +37,32 @@ def test_error_path_without_device_call():
     assert lib.st_create(C.byref(o), None, C.byref(h)) == _lib.ST_E_ARG
     assert b"numParticles" in lib.st_last_error(None)
     o.numParticles = 10
-    # integratorType 6 (the spectral path) is not carried
-    o.integratorType = 6
+    # integratorType 7 does not exist
+    o.integratorType = 7
     assert lib.st_create(C.byref(o), None, C.byref(h)) == _lib.ST_E_UNSUPPORTED
-    assert b"integratorType 6" in lib.st_last_error(None)
+    assert b"integratorType 7" in lib.st_last_error(None)
+    # integratorType 6 (the spectral path) needs a drive frequency and a
+    # sampling step that keeps the sample count bounded; it runs on plain
+    # pipeline handles with spectrum_sum 0 or 1
+    o.integratorType = 6
+    o.w, o.h = 0.0, 1e-4
+    assert lib.st_create(C.byref(o), None, C.byref(h)) == _lib.ST_E_ARG
+    assert b"drive frequency" in lib.st_last_error(None)
+    o.w, o.h = 6283.0, 1e-13
+    assert lib.st_create(C.byref(o), None, C.byref(h)) == _lib.ST_E_ARG
+    assert b"samples every h" in lib.st_last_error(None)
+    o.h = 1e-4
+    x6 = _lib.Ext()
+    x6.pulse_axis, x6.spectrum_sum = 2, 2
+    assert lib.st_create(C.byref(o), C.byref(x6), C.byref(h)) == _lib.ST_E_ARG
+    assert b"spectrum_sum" in lib.st_last_error(None)
+    x6.spectrum_sum, x6.engine = 1, _lib.ENGINE_FUSED
+    assert lib.st_create(C.byref(o), C.byref(x6), C.byref(h)) == _lib.ST_E_UNSUPPORTED
+    x6.engine = _lib.ENGINE_PIPELINE
+    assert lib.st_create_batch(C.byref(o), 1, C.byref(x6), C.byref(h)) == _lib.ST_E_UNSUPPORTED
+    assert b"plain handles" in lib.st_last_error(None)
+    sp = _lib.Spectrum()
+    assert lib.st_spectrum_get(None, C.byref(sp)) == _lib.ST_E_ARG
     # types 2-5 run on the pipeline engine; the fused engine refuses them
     # before touching the device
     o.integratorType = 2
