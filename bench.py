@@ -44,6 +44,10 @@ FLOPS_RK45_TRIAL = 650.0
 # 11 rotation quaternions + 9 products + 2 vector rotations + controller;
 # RKF45 = RK45's count.
 FLOPS_TRIAL = {1: 650.0, 2: 1520.0, 3: 1000.0, 4: 650.0, 5: 1000.0}
+# integratorType 6, per noise sample (the reference's operation order, test
+# noise off): the noise field along a gravity segment 55 FP64 operations, 17
+# Goertzel filters x 3 axes x 11, the sample clock 1
+FLOPS_SAMPLE = 617.0
 FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 vector, dense
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per segment record moved by k_spin (reads + writes), from the
@@ -361,6 +365,58 @@ def bench_integrators(args, variant):
     return {"workload": f"{n} neutrons, spins transverse to B0, one interval per type", "types": out}
 
 
+def bench_spectral(args, variant):
+    """integratorType 6 (noise spectrum + Floquet-Markov end point): He-3 of
+    the headline workload (Maxwell-Boltzmann at 0.45 K, gas collisions with
+    tc = 2e-3 s, 5% diffuse walls, gravity), drive 3e-5 T at 1 kHz along z,
+    the noise field sampled every 1e-4 s into 17 Goertzel filters, 1 s in two
+    output intervals, every particle's spectrum summed; the CPU oracle on a
+    2048-particle sample of the first interval beside it."""
+    import math
+
+    import spintrack as st
+    from spintrack import configs
+
+    n = args.spectral_particles
+
+    def options(m):
+        o = configs.he3_thermal(n=m, tf=1.0, ioutInt=0.5, integrator=6)
+        o.a, o.w, o.h = 3e-5, 2 * math.pi * 1000.0, 1e-4
+        return o
+
+    opt = options(n)
+    targets = st.output_times(opt)
+    with st.Particle(options(min(n, 65536)), spectrum_sum=1, variant=variant) as w:   # warm-up
+        w.initParticles()
+        w.runSimulation(0.01)
+        w.synchronize()
+    with st.Particle(opt, spectrum_sum=1, variant=variant) as p:
+        p.initParticles()
+        p.synchronize()
+        p.kernel_timing(True)
+        p.timer_start()
+        p.run(targets)
+        ms = p.timer_stop()
+        kt = p.kernel_times()
+        wk = p.work_counters()
+        b_end = p.floquetResults()
+        spec = p.spectrum()
+    psec = n * (targets[-1] - opt.t0)
+    samples = wk["accepted"]
+    tfl = FLOPS_SAMPLE * samples / (kt["spin_ms"] / 1e3) / 1e12
+    cpu = cpu_baseline(opt, n, 2048, 1)
+    out = {"workload": f"{n} He-3, integratorType 6, h = 1e-4 s, 17 filters, 1 s in 2 intervals",
+           "variant": args.variant,
+           "value": psec / (ms / 1e3), "unit": "particle-seconds/s", "ms": ms, "kernels": kt,
+           "samples": samples, "samples_per_s": samples / (ms / 1e3),
+           "sampling_kernel_TFLOPs": tfl, "sampling_kernel_frac_fp64_peak": tfl / FP64_PEAK_TFLOPS,
+           "flops_per_sample": FLOPS_SAMPLE, "spectrum_samples": spec["n_samples"],
+           "b_end": [float(v) for v in b_end],
+           "cpu_oracle": cpu["value"], "cpu_sample": cpu["sample"], "gpu_over_cpu": psec / (ms / 1e3) / cpu["value"]}
+    print(f"[bench] spectral: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def bench_dressing(args, variant):
     """Spin dressing: 262,144 neutrons + 262,144 He-3 co-tracked, B0 =
     (1e-6,0,0) T plus 1e-5 T * cos(2 pi 3000 t) along y, gas collisions,
@@ -614,6 +670,8 @@ def main():
     ap.add_argument("--no-reference-runs", action="store_true",
                     help="skip repeating the reference's published runs through the C++ driver")
     ap.add_argument("--integrator-particles", type=int, default=131072)
+    ap.add_argument("--no-spectral", action="store_true", help="skip the integratorType 6 leg")
+    ap.add_argument("--spectral-particles", type=int, default=1048576)
     ap.add_argument("--no-dressing", action="store_true", help="skip the spin-dressing leg")
     ap.add_argument("--dressing-per", type=int, default=262144, help="particles per species")
     ap.add_argument("--no-adjoint-fd", action="store_true", help="skip its finite-difference check")
@@ -703,6 +761,10 @@ def main():
     if not args.no_integrators:
         integ = bench_integrators(args, variant)
 
+    spectral = None
+    if not args.no_spectral:
+        spectral = bench_spectral(args, variant)
+
     dressing = None
     if not args.no_dressing:
         dressing = bench_dressing(args, variant)
@@ -745,6 +807,7 @@ def main():
         "batched_sweep": sweep,
         "rk45": rk45,
         "integrators": integ,
+        "spectral": spectral,
         "snapshot_stream": snap,
         "reference_runs": refruns,
         "dressing": dressing,

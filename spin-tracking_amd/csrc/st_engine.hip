@@ -568,7 +568,9 @@ static int run_pipeline(st_handle* h, RunArgs R) {
     const int integ = h->opt.integratorType;
     const bool spectral = integ == INT_SPECTRUM;
     if (spectral) {
-        if ((r = persistent_grid(h, (const void*)&k_spectrum<SPEC_FB>, &h->grid_spec))) return r;
+        const void* sym = h->ext.variant == ST_VARIANT_FAST ? (const void*)&k_spectrum<SPEC_FB, VAR_FAST>
+                                                            : (const void*)&k_spectrum<SPEC_FB, VAR_EXACT>;
+        if ((r = persistent_grid(h, sym, &h->grid_spec))) return r;
     } else if (integ != INT_NONE &&
                (r = persistent_grid(h, spin_symbol(h->ext.variant, integ, h->batch, h->any_pulse), &h->grid_spin))) {
         return r;
@@ -627,7 +629,12 @@ static int run_pipeline(st_handle* h, RunArgs R) {
                 Ab.R.queue = h->d_small + 4 + b;
                 SpecArgs S = h->spec;
                 S.block = b;
-                hipLaunchKernelGGL(k_spectrum<SPEC_FB>, dim3(grid), dim3(256), 0, h->stream, h->P, h->D, Ab, S);
+                if (h->ext.variant == ST_VARIANT_FAST)
+                    hipLaunchKernelGGL((k_spectrum<SPEC_FB, VAR_FAST>), dim3(grid), dim3(256), 0, h->stream, h->P,
+                                       h->D, Ab, S);
+                else
+                    hipLaunchKernelGGL((k_spectrum<SPEC_FB, VAR_EXACT>), dim3(grid), dim3(256), 0, h->stream, h->P,
+                                       h->D, Ab, S);
                 HIPCHK(h, hipGetLastError());
             }
             if (h->ktime_on) HIPCHK(h, hipEventRecord(h->kev[3], h->stream));
@@ -684,8 +691,7 @@ static int run_fused(st_handle* h, RunArgs R, bool persist) {
 static int spec_setup(st_handle* h) {
     SpecArgs& S = h->spec;
     memset(&S, 0, sizeof S);
-    S.astride = field_stride(h->n) / 8;
-    HIPCHK(h, hipMalloc(&S.agg, sizeof(double) * AGG_FIELDS * S.astride));
+    HIPCHK(h, hipMalloc(&S.agg, sizeof(double) * AGG_FIELDS * 64 * (((size_t)h->n + 63) / 64)));
     HIPCHK(h, hipMalloc(&S.nsamp, sizeof(uint32_t) * h->n));
     HIPCHK(h, hipMalloc(&h->d_spec, sizeof(double) * SPEC_ENTRIES));
     HIPCHK(h, hipMalloc(&h->d_spec_n, sizeof(unsigned long long)));
@@ -708,6 +714,7 @@ static int spec_init(st_handle* h) {
         h->spec.k2[q] = 2 * cos(angle);
         h->spec.cw[q] = cos(angle);
         h->spec.sw[q] = sin(angle);
+        h->spec.sw2[q] = 2 * sin(angle);
     }
     hipLaunchKernelGGL(k_spec_clear, dim3((h->n + 255) / 256), dim3(256), 0, h->stream, h->spec, h->n);
     HIPCHK(h, hipGetLastError());
@@ -1325,12 +1332,12 @@ ST_API int st_spectrum_aggregators(st_handle* h, double* agg, uint32_t* n_sample
     int r = need_spectral(h);
     if (r) return r;
     if (agg) {
-        std::vector<double> dev((size_t)AGG_FIELDS * h->spec.astride);
+        std::vector<double> dev((size_t)AGG_FIELDS * 64 * (((size_t)h->n + 63) / 64));
         HIPCHK(h, hipMemcpyAsync(dev.data(), h->spec.agg, sizeof(double) * dev.size(), hipMemcpyDeviceToHost,
                                  h->stream));
         HIPCHK(h, hipStreamSynchronize(h->stream));
         for (uint32_t i = 0; i < h->n; ++i)
-            for (int f = 0; f < AGG_FIELDS; ++f) agg[(size_t)i * AGG_FIELDS + f] = dev[(size_t)f * h->spec.astride + i];
+            for (int f = 0; f < AGG_FIELDS; ++f) agg[(size_t)i * AGG_FIELDS + f] = dev[agg_index(i, f)];
     }
     if (n_samples) {
         HIPCHK(h, hipMemcpyAsync(n_samples, h->spec.nsamp, sizeof(uint32_t) * h->n, hipMemcpyDeviceToHost, h->stream));

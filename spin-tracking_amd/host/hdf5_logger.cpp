@@ -186,6 +186,9 @@ public:
         field("Num Steps", A_.t_ulong, n_steps);
     }
 
+    // an attribute of the root group, like the options (writeOption<coords>, :68-70)
+    void writeSingle(const char* name, const st_coords& value) override { attr(name, coords_, &value); }
+
 private:
     void attr(const char* name, hid_t type, const void* value) {
         const hid_t sp = A_.Screate(H5S_SCALAR);

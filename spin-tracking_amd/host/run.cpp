@@ -37,6 +37,23 @@ void BinaryLogger::writeSnapshot(const double* t, const st_coords* pos, const st
 
 void BinaryLogger::writeBlock(const void* block, size_t bytes) { std::fwrite(block, 1, bytes, f_); }
 
+void BinaryLogger::writeSingle(const char*, const st_coords& value) { std::fwrite(&value, sizeof value, 1, f_); }
+
+namespace {
+
+// particle::postProcess (src/particle.cpp:845-850): a spectral run
+// (integratorType 6) ends its output with the Floquet result b_end.
+int post_process(st_handle* h, const st_options& opt, Logger& log) {
+    if (opt.integratorType != 6) return ST_OK;
+    double b[3];
+    const int rc = st_floquet_result(h, b);
+    if (rc) return rc;
+    log.writeSingle("b_end", st_coords{b[0], b[1], b[2]});
+    return ST_OK;
+}
+
+}  // namespace
+
 namespace {
 
 struct Handle {
@@ -166,6 +183,7 @@ int main_analysis(st_options opt, const char* output, const RunConfig& cfg, std:
         if ((rc = st_memcpy_d2h(H.h, host.data(), blocks.p, host.size()))) return report(H.h, rc, "st_memcpy_d2h");
         const auto ms = std::chrono::duration_cast<std::chrono::milliseconds>(clk::now() - t0).count();
         for (size_t k = 0; k < times.size(); ++k) log.writeBlock(host.data() + k * bb, bb);
+        if ((rc = post_process(H.h, opt, log))) return report(H.h, rc, "st_floquet_result");
         if (!log.ok()) return report(nullptr, ST_E_ARG, "writing the output file");
         out << "run " << times.size() << " intervals, duration " << (times.empty() ? 0.0 : times.back()) << ", "
             << ms << std::endl;
@@ -195,6 +213,7 @@ int main_analysis(st_options opt, const char* output, const RunConfig& cfg, std:
         }
         writer.drain();
     }
+    if ((rc = post_process(H.h, opt, log))) return report(H.h, rc, "st_floquet_result");
     if (!log.ok()) return report(nullptr, ST_E_ARG, "writing the output file");
     return ST_OK;
 }

@@ -49,6 +49,9 @@ public:
     // t, pos, v, S, failure, n_coll, n_bounce, n_steps); by default split
     // into its arrays and written as a snapshot
     virtual void writeBlock(const void* block, size_t bytes);
+    // one named vector after the snapshots (writeSingle, src/logger.cpp:31-33,
+    // 68-70): the spectral run's b_end
+    virtual void writeSingle(const char* name, const st_coords& value) = 0;
 };
 
 // BinaryLogger (src/logger.cpp:8-38): the 440-byte options block, then one
@@ -66,6 +69,8 @@ public:
                        const uint64_t* n_steps) override;
     // the block is already the file's bytes
     void writeBlock(const void* block, size_t bytes) override;
+    // the raw 24 bytes, no name
+    void writeSingle(const char* name, const st_coords& value) override;
 
 private:
     st_options opt_;

@@ -5,6 +5,7 @@
     for i in 0 .. floor((tf-t0)/ioutInt)-1:
         p.runSimulation((i+1)*ioutInt); p.outputData(log)
         print "iter i, duration nextTime, ms"
+    p.postProcess(log)          # a spectral run appends b_end
 
 ``main_analysis`` keeps that per-interval structure (the drop-in path).
 ``main_analysis_persistent`` produces the same file from one persistent launch
@@ -33,6 +34,7 @@ def main_analysis(opt, output_name: str, pulse_axis: int = 2, variant: int = L.V
             p.outputData(lg)
             ms = int((time.perf_counter() - t0) * 1000)
             print(f"iter {i}, duration {next_t:g}, {ms}", file=log)
+        p.postProcess(lg)
 
 
 def main_analysis_persistent(opt, output_name: str, pulse_axis: int = 2,
@@ -49,3 +51,4 @@ def main_analysis_persistent(opt, output_name: str, pulse_axis: int = 2,
         blocks.free()
         for k in range(len(targets) + 1):
             lg.writeBlock(host[k * bb:(k + 1) * bb])
+        p.postProcess(lg)

@@ -98,3 +98,32 @@ def test_driver_output_matches_python_main_analysis(tmp_path, persistent):
     hdr, snaps = st.read_output(out_cpp)
     assert hdr.numParticles == 192 and snaps["t"].shape == (5, 192)
     assert np.all(snaps["failure"][-1] == 0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("persistent", [False, True])
+def test_driver_spectral_run_appends_b_end(tmp_path, persistent):
+    """integratorType 6 through the driver: the file ends with b_end
+    (particle::postProcess, src/particle.cpp:845-850) and equals the Python
+    mainAnalysis mirror's byte for byte."""
+    text = PARAMS.replace("integratorType, 0", "integratorType, 6").replace("h, 1e-3", "h, 1e-4")
+    text += "a, 3e-5\nw, 6283.185307179586\n"
+    path = write(tmp_path, text)
+    out_cpp = str(tmp_path / "cpp.bin")
+    out_py = str(tmp_path / "py.bin")
+    args = [path, out_cpp] + (["--persistent"] if persistent else [])
+    print(f"[driver] spectral {' '.join(args)}", flush=True)
+    driver(*args)
+    opt = st.read_params(path, log=io.StringIO())
+    assert opt.integratorType == 6 and opt.w > 0
+    if persistent:
+        st.simulation.main_analysis_persistent(opt, out_py, variant=st.VARIANT_EXACT)
+    else:
+        st.simulation.main_analysis(opt, out_py, log=io.StringIO())
+    a, b = open(out_cpp, "rb").read(), open(out_py, "rb").read()
+    assert len(a) == len(b) == 440 + 5 * 108 * 192 + 24
+    assert a == b
+    _, snaps = st.read_output(out_cpp)
+    assert snaps["t"].shape == (5, 192)
+    assert np.all(np.isfinite(snaps["b_end"])) and 0 < np.linalg.norm(snaps["b_end"]) <= 1 + 1e-9
+
