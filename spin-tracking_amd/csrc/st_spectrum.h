@@ -43,7 +43,10 @@ ST_HD __forceinline__ uint64_t agg_index(uint32_t pid, int f) { return agg_tile(
 constexpr int NW_PAD = 18;    // filter constants padded with zeros to whole register blocks
 constexpr int SPEC_FB = 6;    // filters per register block
 constexpr int SPEC_NB = (NW + SPEC_FB - 1) / SPEC_FB;
-constexpr int SPEC_UNROLL = 4;   // samples per loop trip
+// samples per loop trip: eight for the fast variant; the exact variant's
+// longer sample body spills registers beyond four
+template <int VAR>
+constexpr int spec_unroll() { return VAR == VAR_FAST ? 8 : 4; }
 constexpr int SPEC_ENTRIES = NW * 18;   // one interval's spectrum: NW 3x3 complex matrices
 static_assert(4 + SPEC_NB <= 8, "the spectral blocks' queue heads sit in d_small[4..7]");
 static_assert(SPEC_NB * SPEC_FB <= NW_PAD, "padded filter constants cover every block");
@@ -315,10 +318,10 @@ __global__ __launch_bounds__(256, 2) void k_spectrum(const SimParams P, const De
             }
         }
         if (phase == S_SAMPLE) {
-            // up to SPEC_UNROLL samples per trip: the divergent record
+            // up to spec_unroll samples per trip: the divergent record
             // change above is paid once per several samples, not per sample
 #pragma unroll
-            for (int u = 0; u < SPEC_UNROLL; ++u) {
+            for (int u = 0; u < spec_unroll<VAR>(); ++u) {
                 if (u == 0 || ts < lim) {
                     sample(ts);
                     ++ns;
