@@ -9,7 +9,8 @@ restated as known-answer tests elsewhere), so the fixtures are:
 * trajectories: snapshots of small ensembles from the C oracle, the CPU
   restatement pinned by tests/test_oracle_*.py -- a specular neutron run,
   a He-3 run with gas collisions and diffuse walls, and one short run per
-  integratorType 1-5;
+  integratorType 1-6 (type 6 also with its last interval's filter states,
+  the summed spectrum and b_end);
 * integrator calls: in/out tuples of single integrator calls of types 0-5.
 They freeze the checked oracle so that later changes to it, to the device
 code or to the toolchain show up as differences, and give the GPU tests
@@ -17,6 +18,7 @@ reference outputs that need no CPU run.
 
 Run from the repository root:  python tests/golden/make_golden.py
 """
+import math
 import os
 import sys
 
@@ -83,6 +85,11 @@ def trajectory_cases():
         o.rtol = o.atol = rtol
         o.yi.x, o.yi.y, o.yi.z = 0.0, 0.6, 0.8
         cases[f"type{integ}"] = o
+    o = configs.neutron_cell(n=64, tf=0.2, ioutInt=0.1, seed=26)
+    o.integratorType = 6
+    o.a, o.w, o.h = 3e-5, 2 * math.pi * 1000.0, 1e-4
+    o.Gy = st.Coords(2e-9, 0.0, 5e-9)
+    cases["type6"] = o
     return {k: (v, st.output_times(v)) for k, v in cases.items()}
 
 
@@ -96,7 +103,7 @@ def run_oracle(O, opt, targets):
         E.advance_to(t, prev_t=prev)
         prev = t
         snaps.append(E.snapshot())
-    return snaps
+    return snaps, E
 
 
 def integrator_cases(O):
@@ -137,8 +144,14 @@ def main():
     out["rng_outputs"] = np.array(outs, dtype=np.uint64)
     names = []
     for name, (opt, targets) in trajectory_cases().items():
-        snaps = run_oracle(O, opt, targets)
+        snaps, E = run_oracle(O, opt, targets)
         names.append(name)
+        if E.spectral:
+            for k in ("s1", "s2", "im", "n_samples"):
+                out[f"{name}_agg_{k}"] = np.ascontiguousarray(E.agg[k])
+            out[f"{name}_cspec"] = E.cspec
+            out[f"{name}_cspec_n"] = np.array(E.cspec_n, dtype=np.int64)
+            out[f"{name}_b_end"] = E.floquet_result()
         out[f"{name}_options"] = np.frombuffer(to_bytes(opt), dtype=np.uint8)
         out[f"{name}_targets"] = np.asarray(targets, dtype=np.float64)
         for f in ("t", "pos", "v", "S", "failure", "n_coll", "n_bounce", "n_steps"):

@@ -8,6 +8,8 @@ tests/golden/make_golden.py).
   state (bounces, collisions, segments, failures) exactly, t/pos/v/S within
   1e-9 -- for He-3 with gas collisions, at most 2 of 64 particles may leave
   1e-9 through the libm ulp sensitivity measured in test_gpu_parity.py.
+  For integratorType 6 also the filter states, the summed spectrum and
+  b_end.
 """
 import os
 
@@ -39,7 +41,11 @@ def test_rng_fixture(oracle, golden):
         assert np.array_equal(oracle.xoshiro(s, g["rng_outputs"].shape[1]).astype(np.uint64), g["rng_outputs"][k])
 
 
-@pytest.mark.parametrize("name", ["neutrons", "he3", "type1", "type2", "type3", "type4", "type5"])
+CASES = ["neutrons", "he3", "type1", "type2", "type3", "type4", "type5", "type6"]
+AGG = ("s1", "s2", "im", "n_samples")
+
+
+@pytest.mark.parametrize("name", CASES)
 def test_oracle_reproduces_trajectories(oracle, golden, name):
     opt = case_options(golden, name)
     targets = golden[f"{name}_targets"]
@@ -52,6 +58,12 @@ def test_oracle_reproduces_trajectories(oracle, golden, name):
         snaps.append(E.snapshot())
     for f in FIELDS:
         assert np.stack([s[f] for s in snaps]).tobytes() == golden[f"{name}_{f}"].tobytes(), (name, f)
+    if name == "type6":
+        for k in AGG:
+            assert np.ascontiguousarray(E.agg[k]).tobytes() == golden[f"type6_agg_{k}"].tobytes(), k
+        assert E.cspec.tobytes() == golden["type6_cspec"].tobytes()
+        assert E.cspec_n == int(golden["type6_cspec_n"])
+        assert E.floquet_result().tobytes() == golden["type6_b_end"].tobytes()
 
 
 def test_oracle_reproduces_integrator_calls(oracle, golden):
@@ -67,7 +79,7 @@ def test_oracle_reproduces_integrator_calls(oracle, golden):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("variant", [st.VARIANT_EXACT, st.VARIANT_FAST])
-@pytest.mark.parametrize("name", ["neutrons", "he3", "type1", "type2", "type3", "type4", "type5"])
+@pytest.mark.parametrize("name", CASES)
 def test_gpu_matches_golden(golden, name, variant):
     opt = case_options(golden, name)
     targets = golden[f"{name}_targets"]
@@ -78,6 +90,19 @@ def test_gpu_matches_golden(golden, name, variant):
         for t in targets:
             p.runSimulation(float(t))
             snaps.append(p.snapshot())
+        if name == "type6":
+            # filter states, spectrum (the reference CPU path's partial sum) and b_end:
+            # 1e-12 relative for the exact variant, 1e-9 for the fast one
+            tol = 1e-12 if variant == st.VARIANT_EXACT else 1e-9
+            agg, sp, b = p.spectrumAggregators(), p.spectrum(), p.floquetResults()
+            assert np.array_equal(agg["n_samples"].astype(np.int64), golden["type6_agg_n_samples"].astype(np.int64))
+            for k in ("s1", "s2", "im"):
+                ref = golden[f"type6_agg_{k}"]
+                assert np.abs(agg[k] - ref).max() <= tol * np.abs(ref).max(), k
+            assert sp["n_samples"] == int(golden["type6_cspec_n"])
+            ref = golden["type6_cspec"]
+            assert np.abs(sp["raw"] - ref).max() <= tol * np.abs(ref).max()
+            assert np.abs(b - golden["type6_b_end"]).max() < 1e-9
     for f in ("failure", "n_coll", "n_bounce", "n_steps"):
         assert np.array_equal(np.stack([s[f] for s in snaps]), golden[f"{name}_{f}"]), (name, f)
     worst = np.zeros(opt.numParticles)
