@@ -175,3 +175,36 @@ def test_he3_gas_and_diffuse(oracle):
         d = np.abs(g["agg"][k][same] - E.agg[k][same]).max()
         assert d <= 1e-12 * np.abs(E.agg[k][same]).max(), (k, d)
     assert abs(g["spec"]["n_samples"] - E.cspec_n) <= 0.01 * E.cspec_n
+
+
+def test_partial_tile_and_stop_times(oracle):
+    """200 particles (the last 64-particle tile of the filter state is
+    partial) with uneven per-particle end times: particles past their end
+    time sample nothing in later intervals, and their filters are cleared
+    like every other particle's (src/particle.cpp:746-747)."""
+    opt = spectral_neutrons(n=200, tf=0.3)
+    opt.ioutInt = 0.1
+    targets = st.output_times(opt)
+    stop_at = np.random.default_rng(7).uniform(0.05, 0.35, opt.numParticles)
+    print("[spectral] n=200 with stop times", flush=True)
+    for mode in (0, 1):
+        with st.Particle(opt, spectrum_sum=mode) as p:
+            p.initParticles()
+            p.set_stop_times(stop_at)
+            for t in targets:
+                p.runSimulation(t)
+            agg, sp, snap = p.spectrumAggregators(), p.spectrum(), p.snapshot()
+        E = oracle.Ensemble(to_oracle_opt(opt), opt.numParticles, spectrum_sum=mode)
+        prev = opt.t0
+        for t in targets:
+            E.advance_to(t, stop_at=stop_at, prev_t=prev)
+            prev = t
+        r = E.snapshot()
+        assert np.array_equal(snap["n_steps"], r["n_steps"])
+        assert np.array_equal(agg["n_samples"].astype(np.int64), E.agg["n_samples"].astype(np.int64))
+        assert np.all(agg["n_samples"][stop_at <= targets[-2]] == 0)
+        for k in ("s1", "s2", "im"):
+            assert rel(agg[k], E.agg[k]) < 1e-12, k
+        assert sp["n_samples"] == E.cspec_n
+        assert rel(sp["raw"], E.cspec) < 1e-11
+
