@@ -81,7 +81,8 @@ typedef struct st_ext {
                                most 64) in one HIP graph and let st_advance_to / st_run replay it
                                until every particle is done -- one host round trip per replay
                                instead of per chunk; 0 = host-driven chunk loop.  Runs with
-                               checkpointing or per-kernel timing on use the host-driven loop. */
+                               checkpointing or per-kernel timing on, and integratorType 6, use the
+                               host-driven loop. */
     int32_t spectrum_sum;   /* integratorType 6: which particles' covariance spectra each interval adds
                                to the run's spectrum.  0 = the reference CPU path's: its in-place tree
                                (src/particle.cpp:826-834) leaves particles 0 and 2^k < numParticles
