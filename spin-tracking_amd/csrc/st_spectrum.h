@@ -26,9 +26,10 @@
 // then wave-uniform kernel arguments (scalar loads) and a launch's block
 // (6 filters, 108 VGPRs of state) fits the register file without spills at
 // two waves per SIMD; the 18 independent filter chains per lane carry the
-// FP64 latency.  Each sample costs the
-// noise term (~40 FP64 operations, three sines when the test noise is on)
-// plus 33 operations per filter, all in the reference's operation order.
+// FP64 latency.  In the exact variant each sample costs the noise term (55
+// FP64 operations, three sines when the test noise is on) plus 11 operations
+// per filter and axis, in the reference's operation order; the fast variant
+// (below) 7 and 4.
 #pragma once
 
 #include "st_pipeline.h"

@@ -255,6 +255,14 @@ class Particle:
     def synchronize(self) -> None:
         check(self.lib.st_synchronize(self.h), self.h)
 
+    def spinMean(self) -> np.ndarray:
+        """particle::spinMean (src/particle.cpp:878-884): the mean spin over
+        all particles (a stopped particle's NaN spin makes it NaN)."""
+        S = np.zeros(self.n, dtype=COORDS)
+        check(self.lib.st_snapshot(self.h, None, None, None, _ptr(S), None, None, None, None), self.h)
+        # the reference's order, S_sum = S_sum + S[i] (cumsum adds in sequence)
+        return np.cumsum(xyz(S), axis=0)[-1] / self.n
+
     # ---- spectral path (integratorType 6) ----
     def spectrum(self) -> dict:
         """particle::cspec: the summed, un-normalised covariance spectrum."""

@@ -133,6 +133,11 @@ def test_run_equals_advance():
         assert np.array_equal(a["agg"][k], b["agg"][k]), k
     assert np.array_equal(a["spec"]["raw"], b["spec"]["raw"])
     assert np.array_equal(a["b_end"], b["b_end"])
+    # particle::spinMean: S is not integrated, so the mean stays yi
+    with st.Particle(opt) as p:
+        p.initParticles()
+        p.runSimulation(0.1)
+        assert np.allclose(p.spinMean(), [opt.yi.x, opt.yi.y, opt.yi.z], rtol=0, atol=1e-15)
 
 
 def test_sample_grid_invariant():
