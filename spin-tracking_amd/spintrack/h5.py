@@ -107,7 +107,8 @@ def _attr(H, loc, name: str, htype, np_dtype):
 
 def read_hdf5(path: str):
     """Returns (options, dict of arrays shaped (rows written, N)); the
-    options are rebuilt from the root attributes."""
+    options are rebuilt from the root attributes.  A spectral run's b_end
+    root attribute comes back under "b_end"."""
     H = hdf5_lib()
     f = H.L.H5Fopen(os.fsencode(path), 0, 0)   # H5F_ACC_RDONLY, H5P_DEFAULT
     if f < 0:
@@ -145,6 +146,10 @@ def read_hdf5(path: str):
             if rc < 0:
                 raise RuntimeError(f"reading {dname}")
             out[field] = arr[:nt]
+        # a spectral run's final Bloch vector (writeSingle, src/logger.cpp:68-70)
+        if H.L.H5Aexists(f, b"b_end") > 0:
+            v = _attr(H, f, "b_end", H.coords, c3)
+            out["b_end"] = np.array([float(v["x"]), float(v["y"]), float(v["z"])])
         return opt, out
     finally:
         H.L.H5Fclose(f)

@@ -213,3 +213,29 @@ def test_partial_tile_and_stop_times(oracle):
         assert sp["n_samples"] == E.cspec_n
         assert rel(sp["raw"], E.cspec) < 1e-11
 
+
+@pytest.mark.parametrize("variant", [st.VARIANT_EXACT, st.VARIANT_FAST])
+def test_no_gravity_pulse_axis_and_step_switches(oracle, variant):
+    """Gravity off (the fast variant's noise polynomial loses its quadratic
+    term), the drive along y (it enters the Floquet decomposition only) and
+    keepStepSize / fixedStepSize set (no-ops here: the sampling step never
+    changes)."""
+    opt = spectral_neutrons(n=128)
+    opt.gravity = 0
+    opt.keepStepSize = 1
+    opt.fixedStepSize = 1
+    print(f"[spectral] no gravity, pulse on y, step switches, variant={variant}", flush=True)
+    targets = st.output_times(opt)
+    with st.Particle(opt, spectrum_sum=1, variant=variant, pulse_axis=1) as p:
+        p.initParticles()
+        for t in targets:
+            p.runSimulation(t)
+        g = dict(snap=p.snapshot(), agg=p.spectrumAggregators(), spec=p.spectrum(), b_end=p.floquetResults(),
+                 fd=p.floquetInfo(), counters=p.particle_counters())
+    E = oracle.Ensemble(to_oracle_opt(opt), opt.numParticles, spectrum_sum=1, pulse_axis=1)
+    prev = opt.t0
+    for t in targets:
+        E.advance_to(t, prev_t=prev)
+        prev = t
+    check_against_oracle(oracle, opt, 1, g, E, tol=VARIANT_TOL[variant])
+

@@ -84,3 +84,24 @@ def test_driver_hdf5_equals_binary(tmp_path, persistent):
     assert d["t"].shape[0] == b["t"].shape[0] == 5    # initial state + 4 intervals
     fields_equal(d, b)
     assert np.all(np.isfinite(st.xyz(d["S"][-1])))
+
+
+@needs_hdf5
+@pytest.mark.gpu
+def test_driver_hdf5_spectral_b_end(tmp_path):
+    """integratorType 6: the HDF5 file carries b_end as a root attribute
+    (HDF5Logger::writeSingle, src/logger.cpp:68-70), equal to the binary
+    file's trailing b_end."""
+    text = open(write(tmp_path)).read().replace("integratorType, 0", "integratorType, 6")
+    text = text.replace("h, 1e-3", "h, 1e-4") + "a, 3e-5\nw, 6283.185307179586\n"
+    (tmp_path / "p6.txt").write_text(text)
+    params = str(tmp_path / "p6.txt")
+    print("[hdf5] spectral driver run", flush=True)
+    driver(params, str(tmp_path / "run6.bin"))
+    driver(params, str(tmp_path / "run6.h5"))
+    ob, b = logger.read_output(str(tmp_path / "run6.bin"))
+    oh, d = h5.read_hdf5(str(tmp_path / "run6.h5"))
+    assert bytes(oh) == bytes(ob) and oh.integratorType == 6
+    fields_equal(d, b)
+    assert np.array_equal(d["b_end"], b["b_end"]) and np.all(np.isfinite(b["b_end"]))
+
