@@ -272,18 +272,25 @@ def bench_reference_runs(args):
             params = os.path.join(d, name + ".txt")
             with open(params, "w") as f:
                 f.write("".join(f"{k}, {v}\n" for k, v in kv.items()))
-            t0 = time.perf_counter()
-            r = subprocess.run([exe, params, os.path.join(d, name + ".bin")], capture_output=True, text=True,
-                               timeout=900)
-            wall = time.perf_counter() - t0
-            if r.returncode != 0:
-                raise RuntimeError(f"{name}: driver failed: {r.stderr[-500:]}")
-            total_ms = float(r.stdout.strip().splitlines()[-1])
-            os.remove(os.path.join(d, name + ".bin"))
+            def run(*flags):
+                t0 = time.perf_counter()
+                r = subprocess.run([exe, params, os.path.join(d, name + ".bin"), *flags], capture_output=True,
+                                   text=True, timeout=900)
+                wall = time.perf_counter() - t0
+                if r.returncode != 0:
+                    raise RuntimeError(f"{name}: driver failed: {r.stderr[-500:]}")
+                os.remove(os.path.join(d, name + ".bin"))
+                return float(r.stdout.strip().splitlines()[-1]), wall
+
+            # the exact variant (the driver's default) is the like-for-like figure
+            total_ms, wall = run()
             psec = int(kv["numParticles"]) * float(kv["tf"])
             out[name] = {"total_ms": total_ms, "wall_s": wall, "value": psec / (total_ms / 1e3),
                          "reference_total_ms": ref_ms, "reference_value": psec / (ref_ms / 1e3),
                          "speedup": ref_ms / total_ms, "reference_source": cite}
+            if kv["integratorType"] != "-1":
+                fast_ms, _ = run("--fast")
+                out[name].update(fast_total_ms=fast_ms, fast_speedup=ref_ms / fast_ms)
             print(f"[bench] reference run {name}: {out[name]}", file=sys.stderr, flush=True)
     return {"note": "same params.txt as the reference's published runs (its GPU build on an unrecorded "
                     "NVIDIA GPU); end-to-end totals, output file included", "runs": out}
