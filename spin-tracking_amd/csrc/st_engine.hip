@@ -776,7 +776,7 @@ static floquet::Diag read_floquet(const st_floquet* f) {
 
 extern "C" {
 
-ST_API const char* st_version(void) { return "spintrack gfx950 exact,fast dop853,rk45 pipeline,fused"; }
+ST_API const char* st_version(void) { return "spintrack gfx950 exact,fast dop853,rk45,magnus,rk45q,rkf45,rkf45q,spectral pipeline,fused"; }
 
 ST_API const char* st_last_error(const st_handle* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
 
