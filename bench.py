@@ -671,12 +671,12 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiation leg")
     ap.add_argument("--no-rk45", action="store_true", help="skip the RK45 leg")
     ap.add_argument("--rk45-steps", type=int, default=1, help="output intervals of the RK45 leg")
-    ap.add_argument("--rk45-particles", type=int, default=131072,
+    ap.add_argument("--rk45-particles", type=int, default=1048576,
                     help="He-3 atoms of the RK45 leg (the first of the seeded ensemble)")
     ap.add_argument("--no-integrators", action="store_true", help="skip the integratorType 2-5 leg")
     ap.add_argument("--no-reference-runs", action="store_true",
                     help="skip repeating the reference's published runs through the C++ driver")
-    ap.add_argument("--integrator-particles", type=int, default=131072)
+    ap.add_argument("--integrator-particles", type=int, default=1048576)
     ap.add_argument("--no-spectral", action="store_true", help="skip the integratorType 6 leg")
     ap.add_argument("--spectral-particles", type=int, default=1048576)
     ap.add_argument("--no-dressing", action="store_true", help="skip the spin-dressing leg")
