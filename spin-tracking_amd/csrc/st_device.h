@@ -145,21 +145,46 @@ ST_HD inline double normal(Rng& s, double mean, double sd) {
     return mean + sd * tu * ts;
 }
 ST_HD __forceinline__ double maxboltz(Rng& s, double sqrtkT_m) { return normal(s, 0.0, 1.0) * sqrtkT_m; }
-// Three consecutive normal(0,1) draws through ONE copy of the polar-method
-// code (a rolled loop keeps the event handler small).  normal(s, 0, 1)
-// returns 0 + (1*tu)*ts or spare*1 + 0, i.e. exactly tu*ts or spare.
+// One polar-method pair, as normal() makes it when no spare is cached: the
+// returned draw 0 + (1*tu)*ts (exactly tu*ts) and the new spare tv*ts.
+ST_HD __forceinline__ void polar_pair(Rng& s, double& first, double& second) {
+    double ts, tu, tv;
+    do {
+        tu = uni_lh(s, -1.0, 1.0);
+        tv = uni_lh(s, -1.0, 1.0);
+        ts = tu * tu + tv * tv;
+    } while (ts >= 1.0 || ts == 0.0);
+    ts = sqrt(-2.0 * log(ts) / ts);
+    first = 0.0 + 1.0 * tu * ts;
+    second = tv * ts;
+}
+// Three consecutive normal(s, 0, 1) draws.  With a spare cached they are
+// spare, then one new pair (first, second); without, one pair and the first
+// value of a second pair whose other value stays cached.  So a lane needs
+// one or two pairs, always in the same order from its stream, and a wave runs
+// the polar-method code twice at most (three calls of normal() would run it
+// three times whenever the wave's lanes differ in their cached spare).  A
+// spare is returned as spare*1 + 0, i.e. exactly the spare.
 ST_HD __forceinline__ void normal3(Rng& s, double& n0, double& n1, double& n2) {
-    double out0 = 0.0, out1 = 0.0, out2 = 0.0;
-#pragma unroll 1
-    for (int c = 0; c < 3; ++c) {
-        const double g = normal(s, 0.0, 1.0);
-        if (c == 0) out0 = g;
-        else if (c == 1) out1 = g;
-        else out2 = g;
+    const bool had = s.has;
+    const double sp = s.spare;
+    double a0, a1;
+    polar_pair(s, a0, a1);
+    if (had) {
+        n0 = sp * 1.0 + 0.0;
+        n1 = a0;
+        n2 = a1 * 1.0 + 0.0;
+        s.spare = a1;
+        s.has = false;
+    } else {
+        double b0, b1;
+        polar_pair(s, b0, b1);
+        n0 = a0;
+        n1 = a1 * 1.0 + 0.0;
+        n2 = b0;
+        s.spare = b1;
+        s.has = true;
     }
-    n0 = out0;
-    n1 = out1;
-    n2 = out2;
 }
 ST_HD __forceinline__ double unif02pi(Rng& s) { return uni(s) * 2.0 * PI_D; }
 ST_HD __forceinline__ double expo_draw(Rng& s, double tc) { return -tc * log(1.0 - uni(s)); }
@@ -212,26 +237,40 @@ ST_HD inline void next_collision(const SimParams& P, const Phys& F, double tf, K
         dz = psgn(v.z) * P.Lz / 2.0 - pos.z;
         dtx = dx / v.x;
         dtz = dz / v.z;
+        // The reference's three cases (falling: floor; rising below the
+        // ceiling: apex then floor; rising to the ceiling) each take one
+        // square root.  Here the cases pick the root's argument and share
+        // one sqrt, and the falling and ceiling cases share their two
+        // divisions: the ceiling case's (-v.y + sqr) and (-v.y - sqr) are
+        // the falling case's (sqr - v.y) and -(sqr + v.y) up to the sign of
+        // a zero, which fabs drops, and its std_min operands are passed in
+        // its own order.  A wave whose lanes fall and rise then runs one
+        // root and two divisions instead of one set per case.
         const double y2 = v.y * v.y;
+        double arg, t1 = 0.0;
+        bool apex = false, ceiling = false;
         if (psgn(v.y) <= 0.0) {
             dy = pos.y + P.Ly * 0.5;
-            const double sqr = sqrt(-2.0 * G_CONST * dy + y2);
-            const double temp1 = -(sqr + v.y) / G_CONST;
-            const double temp2 = (sqr - v.y) / G_CONST;
-            dty = std_min(fabs(temp1), fabs(temp2));
+            arg = -2.0 * G_CONST * dy + y2;
         } else {
             const double maxHeight = -0.5 * y2 / G_CONST + pos.y;
             if (maxHeight < 0.5 * P.Ly) {
-                const double t1 = -v.y / G_CONST;
+                apex = true;
+                t1 = -v.y / G_CONST;
                 const double top = pos.y + v.y * t1 + 0.5 * G_CONST * t1 * t1;
-                const double t2 = sqrt((-0.5 * P.Ly - top) * 2.0 / G_CONST);
-                dty = t1 + t2;
+                arg = (-0.5 * P.Ly - top) * 2.0 / G_CONST;
             } else {
-                const double sqr = sqrt(y2 - 2.0 * G_CONST * (pos.y - (0.5 * P.Ly)));
-                const double temp1 = (-v.y + sqr) / G_CONST;
-                const double temp2 = (-v.y - sqr) / G_CONST;
-                dty = std_min(fabs(temp1), fabs(temp2));
+                ceiling = true;
+                arg = y2 - 2.0 * G_CONST * (pos.y - (0.5 * P.Ly));
             }
+        }
+        const double sqr = sqrt(arg);
+        if (apex) {
+            dty = t1 + sqr;
+        } else {
+            const double tm = -(sqr + v.y) / G_CONST;   // ceiling case: (-v.y - sqr) / G
+            const double tp = (sqr - v.y) / G_CONST;    // ceiling case: (-v.y + sqr) / G
+            dty = ceiling ? std_min(fabs(tp), fabs(tm)) : std_min(fabs(tm), fabs(tp));
         }
         if (dtx < 1e-16 || __builtin_isnan(dtx))
             dtx = 1e6;
@@ -349,7 +388,11 @@ ST_HD inline void move_and_collide(const SimParams& P, const Phys& F, Kin& K, Rn
             // draws phi and theta before it looks at the wall id.
             const double phi = acos(sqrt(uni(rng)));
             const double theta = unif02pi(rng);
-            const double sp = sin(phi), cp = cos(phi), cth = cos(theta), sth = sin(theta);
+            // sincos shares one argument reduction between the two values
+            // (each equals the separate sin / cos of the device libm)
+            double sp, cp, sth, cth;
+            sincos(phi, &sp, &cp);
+            sincos(theta, &sth, &cth);
             if (wh == 'x') {
                 K.v.x = -1.0 * psgn(K.v.x) * Vel * cp;
                 K.v.y = -Vel * sp * cth;
