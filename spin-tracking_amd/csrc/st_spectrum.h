@@ -221,7 +221,12 @@ __global__ __launch_bounds__(256, 2) void k_spectrum(const SimParams P, const De
     // SpectrumAggregator::update(B/2) at time t: goertzel_stage_1, then
     // goertzel_stage_2_vector of the new state, then covMat::add_outer(a, b,
     // x, 0) on imag_diag
-    auto sample = [&](double t) {
+    // One sample with the Goertzel state held as (sa = s1, sb = s2): the new
+    // s1 is written over sb, after which sb holds s1 and sa holds s2.  The
+    // trip loop calls it with the two arrays' roles alternating, so the state
+    // rotation s2 = s1, s1 = new costs no register copies (under a lane mask
+    // each copy is a real move: four 32-bit moves per filter and axis).
+    auto sample = [&](double t, double (&sa)[FB][3], double (&sb)[FB][3]) {
         double x[3];
         if constexpr (VAR == VAR_FAST) {
             const double tau = t - seg.t0;
@@ -246,20 +251,16 @@ __global__ __launch_bounds__(256, 2) void k_spectrum(const SimParams P, const De
                 const double k = ck[f], sw2 = cc[f];
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    const double n1 = fma(k, s1[f][c], x[c]) - s2[f][c];
-                    s2[f][c] = s1[f][c];
-                    s1[f][c] = n1;
-                    ac[f][c] = fma(sw2 * s2[f][c], x[c], ac[f][c]);
+                    sb[f][c] = fma(k, sa[f][c], x[c]) - sb[f][c];
+                    ac[f][c] = fma(sw2 * sa[f][c], x[c], ac[f][c]);
                 }
             } else {
                 const double k = ck[f], cw = cc[f], sw = cs[f];
 #pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    const double n1 = (x[c] + k * s1[f][c]) - s2[f][c];
-                    s2[f][c] = s1[f][c];
-                    s1[f][c] = n1;
-                    const double re = s1[f][c] - cw * s2[f][c];
-                    const double im = sw * s2[f][c];
+                    sb[f][c] = (x[c] + k * sa[f][c]) - sb[f][c];
+                    const double re = sb[f][c] - cw * sa[f][c];
+                    const double im = sw * sa[f][c];
                     ac[f][c] = ac[f][c] + 2 * (im * x[c] - re * 0.0);
                 }
             }
@@ -321,17 +322,40 @@ __global__ __launch_bounds__(256, 2) void k_spectrum(const SimParams P, const De
         if (phase == S_SAMPLE) {
             // up to spec_unroll samples per trip: the divergent record
             // change above is paid once per several samples, not per sample
+            // Samples alternate the roles of s1 and s2 (even: s1 = sa, odd:
+            // s1 = sb).  A lane that stops sampling in this trip stays stopped
+            // until its end, so after an odd number of samples its two arrays
+            // are swapped back once here.
+            const uint32_t ns0 = ns;
+            auto advance = [&]() {
+                ++ns;
+                const double tn = ts + L.tempH;
+                // (a step below the rounding of ts would never end the
+                // segment; the host bounds h so that this cannot happen)
+                if (tn == ts) lim = ts;
+                ts = tn;
+            };
+            static_assert(spec_unroll<VAR>() % 2 == 0, "samples are taken in pairs of roles");
 #pragma unroll
-            for (int u = 0; u < spec_unroll<VAR>(); ++u) {
+            for (int u = 0; u < spec_unroll<VAR>(); u += 2) {
                 if (u == 0 || ts < lim) {
-                    sample(ts);
-                    ++ns;
-                    const double tn = ts + L.tempH;
-                    // (a step below the rounding of ts would never end the
-                    // segment; the host bounds h so that this cannot happen)
-                    if (tn == ts) lim = ts;
-                    ts = tn;
+                    sample(ts, s1, s2);
+                    advance();
                 }
+                if (ts < lim) {
+                    sample(ts, s2, s1);
+                    advance();
+                }
+            }
+            if ((ns - ns0) & 1u) {
+#pragma unroll
+                for (int f = 0; f < FB; ++f)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const double a = s1[f][c];
+                        s1[f][c] = s2[f][c];
+                        s2[f][c] = a;
+                    }
             }
             if (!(ts < lim)) phase = S_NEXT;
         }
