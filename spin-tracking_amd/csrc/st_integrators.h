@@ -631,15 +631,19 @@ ST_HD __forceinline__ Quat rodriguez_quat(const d3& k, double dt) {
     const double angle = len3(k);
     const double nx = k.x / angle, ny = k.y / angle, nz = k.z / angle;
     const double hh = angle * dt;
-    const double s = -sin(hh / 2.0);
-    return Quat{cos(hh / 2.0), nx * s, ny * s, nz * s};
+    // sincos: one argument reduction for both values (each equals the
+    // device libm's separate sin / cos)
+    double sn, cs;
+    sincos(hh / 2.0, &sn, &cs);
+    const double s = -sn;
+    return Quat{cs, nx * s, ny * s, nz * s};
 }
 
 // v rotated by |k| about -k/|k| (:158-163)
 ST_HD __forceinline__ d3 rodriguez(const d3& k, const d3& v) {
     const double angle = len3(k);
-    const double s = sin(angle);
-    const double c = cos(angle);
+    double s, c;
+    sincos(angle, &s, &c);   // one argument reduction (values as sin, cos)
     const d3 vk = cross3(v, k);
     const double sa = s / angle;
     const double f = ((k.x * v.x + k.y * v.y + k.z * v.z) * (1.0 - c)) / (angle * angle);
