@@ -650,6 +650,61 @@ ST_HD __forceinline__ d3 rodriguez(const d3& k, const d3& v) {
     return d3{(v.x * c + vk.x * sa) + k.x * f, (v.y * c + vk.y * sa) + k.y * f, (v.z * c + vk.z * sa) + k.z * f};
 }
 
+// Fast variant of the two rotations, without square root, division or trig
+// call for angles up to 2: cos x and sin(x)/x for x^2 <= 1 from their Taylor
+// series in x^2 (terms to x^20 and x^18; the first omitted terms are below
+// 1e-19 of the sum).  A Rodrigues rotation by theta = |k| takes them at the
+// half angle x = theta/2: sin(theta)/theta = s c, (1 - cos theta)/theta^2 =
+// s^2/2 (no cancellation) and cos theta = 1 - theta^2 s^2/2.  A quaternion
+// takes them at its own half angle.  Other vectors (larger, zero: the
+// reference's 0/0, or not finite) take the exact form.  The values agree
+// with the exact form to a few units in the last place, well inside the
+// integrator tolerance the fast variant is held to.
+ST_HD __forceinline__ void half_series(double x2, double& c, double& s) {
+#pragma clang fp contract(fast)
+    c = 1.0 + x2 * (-1.0 / 2 + x2 * (1.0 / 24 + x2 * (-1.0 / 720 + x2 * (1.0 / 40320 + x2 * (-1.0 / 3628800 +
+        x2 * (1.0 / 479001600 + x2 * (-1.0 / 87178291200.0 + x2 * (1.0 / 20922789888000.0 +
+        x2 * (-1.0 / 6402373705728000.0 + x2 * (1.0 / 2432902008176640000.0))))))))));
+    s = 1.0 + x2 * (-1.0 / 6 + x2 * (1.0 / 120 + x2 * (-1.0 / 5040 + x2 * (1.0 / 362880 + x2 * (-1.0 / 39916800 +
+        x2 * (1.0 / 6227020800.0 + x2 * (-1.0 / 1307674368000.0 + x2 * (1.0 / 355687428096000.0 +
+        x2 * (-1.0 / 121645100408832000.0)))))))));
+}
+template <int VAR>
+ST_HD __forceinline__ d3 rodriguez_v(const d3& k, const d3& v) {
+    if constexpr (VAR == VAR_FAST) {
+#pragma clang fp contract(fast)
+        const double t2 = k.x * k.x + k.y * k.y + k.z * k.z;
+        if (t2 > 0.0 && t2 <= 4.0) {
+            double ch, sh;
+            half_series(0.25 * t2, ch, sh);
+            const double sa = sh * ch;           // sin(theta) / theta
+            const double f1 = 0.5 * (sh * sh);   // (1 - cos theta) / theta^2
+            const double c = 1.0 - t2 * f1;      // cos theta
+            const d3 vk = cross3(v, k);
+            const double f = (k.x * v.x + k.y * v.y + k.z * v.z) * f1;
+            return d3{v.x * c + vk.x * sa + k.x * f, v.y * c + vk.y * sa + k.y * f, v.z * c + vk.z * sa + k.z * f};
+        }
+    }
+    return rodriguez(k, v);
+}
+// the quaternion of a rotation by |k| dt: half angle p = |k| dt / 2,
+// (cos p, -k (dt/2) sin(p)/p)
+template <int VAR>
+ST_HD __forceinline__ Quat rodriguez_quat_v(const d3& k, double dt) {
+    if constexpr (VAR == VAR_FAST) {
+#pragma clang fp contract(fast)
+        const double hd = 0.5 * dt;
+        const double p2 = (k.x * k.x + k.y * k.y + k.z * k.z) * (hd * hd);
+        if (p2 > 0.0 && p2 <= 1.0) {
+            double c, sp;
+            half_series(p2, c, sp);
+            const double s = -(hd * sp);
+            return Quat{c, k.x * s, k.y * s, k.z * s};
+        }
+    }
+    return rodriguez_quat(k, dt);
+}
+
 // Commutator-free exponential integrators of orders 8 and 6 (Alvermann &
 // Fehske, J. Comput. Phys. 230 (2011) 5930), gamma*B sampled at the 5
 // Gauss-Legendre nodes; include/coeff.h:258-321 (quadrature weights folded in).
@@ -820,13 +875,13 @@ ST_HD inline int rk2_trial(const SimParams& P, const Phys& Fp, RkState& R, d3& y
         // factor and uses weights E_i (RK45: W = CH, E = C; RKF45: W = E = C)
         const double(*Q)[3] = QUAT_TAB[INTEG == INT_RKF45Q ? 1 : 0];
         d3 k = W(t);
-        Quat qw = rodriguez_quat(k, h * Q[0][1]);
-        Quat qe = rodriguez_quat(k, h * Q[0][2]);
+        Quat qw = rodriguez_quat_v<VAR>(k, h * Q[0][1]);
+        Quat qe = rodriguez_quat_v<VAR>(k, h * Q[0][2]);
 #pragma unroll 1
         for (int i = 1; i < 6; ++i) {
             k = W(t + Q[i][0] * h);
-            qw = q_mult(rodriguez_quat(k, h * Q[i][1]), qw);
-            if (i < 5) qe = q_mult(rodriguez_quat(k, h * Q[i][2]), qe);
+            qw = q_mult(rodriguez_quat_v<VAR>(k, h * Q[i][1]), qw);
+            if (i < 5) qe = q_mult(rodriguez_quat_v<VAR>(k, h * Q[i][2]), qe);
         }
         const d3 ws = qv_mult(qw, y);
         const d3 te = qv_mult(qe, y);
@@ -846,10 +901,10 @@ ST_HD inline int rk2_trial(const SimParams& P, const Phys& Fp, RkState& R, d3& y
 #pragma unroll 1
         for (int r = 0; r < 11; ++r) {
             const double* c = CFET_ROWS[r];
-            y8 = rodriguez(cfet_exp(c[0], c[1], c[2], c[3], c[4], B, h), y8);
+            y8 = rodriguez_v<VAR>(cfet_exp(c[0], c[1], c[2], c[3], c[4], B, h), y8);
             if (r < 5) {
                 const double* c6 = CFET_ROWS[11 + r];
-                y6 = rodriguez(cfet_exp(c6[0], c6[1], c6[2], c6[3], c6[4], B, h), y6);
+                y6 = rodriguez_v<VAR>(cfet_exp(c6[0], c6[1], c6[2], c6[3], c6[4], B, h), y6);
             }
         }
         const d3 d{y8.x - y6.x, y8.y - y6.y, y8.z - y6.z};
