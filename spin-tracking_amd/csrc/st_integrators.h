@@ -651,12 +651,13 @@ ST_HD __forceinline__ d3 rodriguez(const d3& k, const d3& v) {
 }
 
 // Fast variant of the two rotations, without square root, division or trig
-// call for angles up to 2: cos x and sin(x)/x for x^2 <= 1 from their Taylor
-// series in x^2 (terms to x^20 and x^18; the first omitted terms are below
-// 1e-19 of the sum).  A Rodrigues rotation by theta = |k| takes them at the
-// half angle x = theta/2: sin(theta)/theta = s c, (1 - cos theta)/theta^2 =
-// s^2/2 (no cancellation) and cos theta = 1 - theta^2 s^2/2.  A quaternion
-// takes them at its own half angle.  Other vectors (larger, zero: the
+// call: cos x and sin(x)/x for x^2 <= 1 from their Taylor series in x^2
+// (terms to x^20 and x^18; the first omitted terms are below 1e-19 of the
+// sum).  A Rodrigues rotation by theta = |k| <= 4 takes them at x = theta/4
+// and doubles twice: for an angle 2x, sin(2x)/(2x) = s c,
+// (1 - cos 2x)/(2x)^2 = s^2/2 (no cancellation) and cos 2x = 1 - (2x)^2 s^2/2
+// (within 6 units in the last place of libm over the range).  A quaternion
+// takes them at its own half angle, up to 1.  Other vectors (larger, zero: the
 // reference's 0/0, or not finite) take the exact form.  The values agree
 // with the exact form to a few units in the last place, well inside the
 // integrator tolerance the fast variant is held to.
@@ -674,9 +675,12 @@ ST_HD __forceinline__ d3 rodriguez_v(const d3& k, const d3& v) {
     if constexpr (VAR == VAR_FAST) {
 #pragma clang fp contract(fast)
         const double t2 = k.x * k.x + k.y * k.y + k.z * k.z;
-        if (t2 > 0.0 && t2 <= 4.0) {
-            double ch, sh;
-            half_series(0.25 * t2, ch, sh);
+        if (t2 > 0.0 && t2 <= 16.0) {
+            // series at theta/4, doubled to theta/2, then to theta
+            double cq, sq;
+            half_series(0.0625 * t2, cq, sq);
+            const double sh = sq * cq;                            // sin(theta/2) / (theta/2)
+            const double ch = 1.0 - (0.25 * t2) * (0.5 * (sq * sq));   // cos(theta/2)
             const double sa = sh * ch;           // sin(theta) / theta
             const double f1 = 0.5 * (sh * sh);   // (1 - cos theta) / theta^2
             const double c = 1.0 - t2 * f1;      // cos theta
