@@ -17,7 +17,12 @@
  * of the last failure is available from st_last_error().  Per-particle
  * failures keep the reference's convention (failureState 1-4 from the
  * sanity check, negative integrator codes, NaN-poisoned t/pos/v/S;
- * src/particle.cpp:412-444, 596-612).  All pointers passed to the st_*
+ * src/particle.cpp:412-444, 596-612), plus one code of this library's own:
+ * ST_FAIL_WATCHDOG (-9), a particle stopped after st_ext.max_events events
+ * in one output interval (the reference's loop has no such bound; a lane
+ * whose event loop does not end would otherwise hold the GPU).  Integrator
+ * codes: DOP853 -1 (nmax), RK45 / RKF45 / quaternion / Magnus -1 (h < hmin)
+ * and -2 (nmax) (src/integrator.cpp:129-131, 280-286).  All pointers passed to the st_*
  * calls are host pointers unless the name ends in _device.  One handle must
  * not be used from two host threads at once (the reference's particle
  * object is not re-entrant either).
@@ -40,6 +45,14 @@ extern "C" {
 #define ST_E_HIP        -2   /* a HIP runtime call failed */
 #define ST_E_UNSUPPORTED -3  /* an option this build does not implement */
 #define ST_E_STATE      -4   /* call out of order (e.g. advance before init) */
+
+/* per-particle failureState of a particle stopped by the event watchdog */
+#define ST_FAIL_WATCHDOG -9
+/* default per-interval event bound (st_ext.max_events == 0) */
+#define ST_DEFAULT_MAX_EVENTS 100000000u
+/* largest ensemble (plain or batched) one handle holds: lanes address their
+ * particle's state with 32-bit byte offsets (8 bytes per particle) */
+#define ST_MAX_PARTICLES (1u << 29)
 
 /* include/double3.h:17-21 */
 typedef struct st_coords { double x, y, z; } st_coords;
@@ -88,6 +101,9 @@ typedef struct st_ext {
                                (src/particle.cpp:826-834) leaves particles 0 and 2^k < numParticles
                                in slot 0, added in that order; 1 = every particle (aggregateSpectrum,
                                src/particle.cpp:893-897), summed in a fixed order on the device */
+    uint32_t max_events;    /* events one particle may take in one output interval before it is
+                               stopped with failureState ST_FAIL_WATCHDOG; 0 = ST_DEFAULT_MAX_EVENTS,
+                               0xffffffff = no bound (the reference's loop) */
 } st_ext;
 
 /* include/particle.h:52-59 */

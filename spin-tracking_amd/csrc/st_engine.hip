@@ -36,6 +36,7 @@
 #include "st_spectrum.h"
 
 static_assert(sizeof(st_options) == 440, "st_options must match the reference's 440-byte options");
+static_assert(st::FAIL_WATCHDOG == ST_FAIL_WATCHDOG, "watchdog failure code");
 static_assert(offsetof(st_options, nmax) == 360, "st_options layout");
 static_assert(offsetof(st_options, dist) == 384, "st_options layout");
 static_assert(offsetof(st_options, noiseAmplitudes) == 392, "st_options layout");
@@ -440,7 +441,7 @@ static RunArgs base_args(st_handle* h) {
     R.queue = h->d_small;
     R.stop_at = h->have_stop_at ? h->d_stop_at : nullptr;
     R.block_bytes = (uint64_t)108 * h->n;
-    R.max_events = 100000000u;
+    R.max_events = h->ext.max_events ? h->ext.max_events : ST_DEFAULT_MAX_EVENTS;
     if (h->trace_pid >= 0 && h->d_trace) {
         R.trace = h->d_trace;
         R.trace_count = h->d_trace_count;
@@ -800,6 +801,8 @@ static int check_ext(const st_ext* ext, st_ext& x) {
 
 static int check_options(const st_options& o) {
     if (o.numParticles <= 0) return fail(nullptr, ST_E_ARG, "numParticles must be > 0 (got %d)", o.numParticles);
+    if ((uint32_t)o.numParticles > ST_MAX_PARTICLES)
+        return fail(nullptr, ST_E_ARG, "numParticles must be at most %u (got %d)", ST_MAX_PARTICLES, o.numParticles);
     if (o.integratorType > 6 || o.integratorType < -1)
         return fail(nullptr, ST_E_UNSUPPORTED,
                     "integratorType %d is not implemented (0 = DOP853, 1 = RK45, 2 = MagnusCFET, 3 = RK45 "
@@ -926,7 +929,9 @@ ST_API int st_create_batch(const st_options* cfgs, int32_t n_cfg, const st_ext* 
         seeds[c] = cfgs[c].seed;
         phys[c] = derive_phys(cfgs[c], x);
     }
-    if (total > 0x7fffffffull) return fail(nullptr, ST_E_ARG, "too many particles in the batch");
+    if (total > ST_MAX_PARTICLES)
+        return fail(nullptr, ST_E_ARG, "too many particles in the batch (%llu, at most %u)", (unsigned long long)total,
+                    ST_MAX_PARTICLES);
     off[n_cfg] = (uint32_t)total;
     st_handle* h = nullptr;
     if ((r = create_common(cfgs[0], x, (uint32_t)total, &h))) return r;

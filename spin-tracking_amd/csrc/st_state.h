@@ -118,7 +118,7 @@ struct RunArgs {
     uint32_t trace_cap;
 };
 
-constexpr int32_t FAIL_WATCHDOG = -9;
+constexpr int32_t FAIL_WATCHDOG = -9;   // ST_FAIL_WATCHDOG of the C ABI
 
 __device__ __forceinline__ double target_of(const RunArgs& R, int32_t k) {
     return R.targets ? R.targets[k] : R.t_single;

@@ -17,6 +17,10 @@ ST_E_HIP = -2
 ST_E_UNSUPPORTED = -3
 ST_E_STATE = -4
 
+FAIL_WATCHDOG = -9            # failureState of a particle stopped by the event watchdog
+DEFAULT_MAX_EVENTS = 100000000
+MAX_PARTICLES = 1 << 29       # particles per handle (32-bit lane offsets)
+
 VARIANT_EXACT = 0
 VARIANT_FAST = 1
 
@@ -59,7 +63,7 @@ ENGINE_FUSED = 1
 class Ext(C.Structure):
     _fields_ = [("pulse_axis", C.c_int32), ("variant", C.c_int32), ("block_size", C.c_int32),
                 ("engine", C.c_int32), ("seg_buffer_mb", C.c_int32), ("chunk_records", C.c_int32),
-                ("graph_chunks", C.c_int32), ("spectrum_sum", C.c_int32)]
+                ("graph_chunks", C.c_int32), ("spectrum_sum", C.c_int32), ("max_events", C.c_uint32)]
 
 
 class Rng(C.Structure):

@@ -48,13 +48,14 @@ def test_neutrons_match_oracle(oracle, integ, variant):
         print(f"  persistent={persistent}: {res}", flush=True)
 
 
+@pytest.mark.parametrize("variant", [st.VARIANT_EXACT, st.VARIANT_FAST])
 @pytest.mark.parametrize("integ", [2, 3])
-def test_he3_with_gas_matches_oracle(oracle, integ):
+def test_he3_with_gas_matches_oracle(oracle, integ, variant):
     opt = configs.he3_thermal(n=256, tf=0.1, ioutInt=0.05, integrator=integ)
     opt.yi.x, opt.yi.y, opt.yi.z = 0.0, 1.0, 0.0
     targets = st.output_times(opt)
-    print(f"[integrators] he3 {NAMES[integ]}", flush=True)
-    g, gs, gc = run_gpu(opt, targets)
+    print(f"[integrators] he3 {NAMES[integ]} variant={variant}", flush=True)
+    g, gs, gc = run_gpu(opt, targets, variant=variant)
     E = run_oracle(oracle, opt, targets)
     print(" ", compare_chaotic(oracle, opt, targets, g, gs, gc, E), flush=True)
 

@@ -91,12 +91,12 @@ def compare(O, g, gstate, gcnt, E, tol=TOL, step_slack=0):
     return out
 
 
-def ulp_control(O, opt, targets):
+def ulp_control(O, opt, targets, pulse_axis=2):
     """The reference algorithm against itself with each particle's first gas
     time moved by one ulp; returns per-particle max |dS| and max |dpos|."""
     runs = []
     for perturb in (False, True):
-        E = O.Ensemble(to_oracle_opt(opt), opt.numParticles)
+        E = O.Ensemble(to_oracle_opt(opt), opt.numParticles, pulse_axis=pulse_axis)
         if perturb:
             E.arr["next_gas"][:] = np.nextafter(E.arr["next_gas"], np.inf)
         prev = opt.t0
@@ -108,11 +108,11 @@ def ulp_control(O, opt, targets):
     return (np.abs(O.xyz(a["S"]) - O.xyz(b["S"])).max(1), np.abs(O.xyz(a["pos"]) - O.xyz(b["pos"])).max(1))
 
 
-def compare_chaotic(O, opt, targets, g, gs, gc, E):
+def compare_chaotic(O, opt, targets, g, gs, gc, E, pulse_axis=2):
     r = check_integer_state(O, g, gs, gc, E)
     dS = np.abs(st.xyz(g["S"]) - O.xyz(r["S"])).max(1)
     dp = np.abs(st.xyz(g["pos"]) - O.xyz(r["pos"])).max(1)
-    cS, cp = ulp_control(O, opt, targets)
+    cS, cp = ulp_control(O, opt, targets, pulse_axis)
     out = dict(frac_dS_gt_1e9=float(np.mean(dS > TOL)), ctrl_frac_dS=float(np.mean(cS > TOL)),
                frac_dpos_gt_1e9=float(np.mean(dp > TOL)), ctrl_frac_dpos=float(np.mean(cp > TOL)),
                median_dS=float(np.median(dS)), max_dS=float(dS.max()), ctrl_max_dS=float(cS.max()))
