@@ -13,6 +13,7 @@ The structured numpy dtypes below mirror the C structs byte for byte:
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 import subprocess
@@ -118,19 +119,35 @@ def build() -> None:
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
-_lib = None
+_libs = {}
+_current = LIB_PATH
 
 # integratorType -> oracle function
 INTEGRATORS = ("or_integrate_dop", "or_integrate_rk45", "or_integrate_magnus",
                "or_integrate_rk45q", "or_integrate_rkf45", "or_integrate_rkf45q")
 
 
+@contextlib.contextmanager
+def library(path: str):
+    """Runs the oracle functions (and Ensembles created) inside the block
+    with another build of the oracle, e.g. one compiled with other field
+    hooks (oracle/Makefile FIELDS=...)."""
+    global _current
+    old, _current = _current, path
+    try:
+        yield lib()
+    finally:
+        _current = old
+
+
 def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
+    path = _current
+    if path not in _libs:
+        if not os.path.exists(path):
+            if path != LIB_PATH:
+                raise FileNotFoundError(path)
             build()
-        L = C.CDLL(LIB_PATH)
+        L = C.CDLL(path)
         vp, d, u32, i32 = C.c_void_p, C.c_double, C.c_uint32, C.c_int
         L.or_xoshiro.restype = C.c_uint64
         L.or_xoshiro.argtypes = [vp]
@@ -180,8 +197,8 @@ def lib():
         L.or_sizeof_specagg.restype = i32
         L.or_sizeof_options.restype = i32
         L.or_sizeof_rng.restype = i32
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def _ptr(a: np.ndarray) -> int:
@@ -380,7 +397,8 @@ class Ensemble:
         self.n_ode = np.zeros(3 * self.n, dtype=np.uint64)
         self._a = _Arrays(**{k: _ptr(v) for k, v in self.arr.items()})
         self._ext = _ext(pulse_axis)
-        lib().or_init_particles(_ptr(self.opt), C.addressof(self._a), self.n, first, stride)
+        self._lib = lib()
+        self._lib.or_init_particles(_ptr(self.opt), C.addressof(self._a), self.n, first, stride)
         self.spectral = int(self.opt["integratorType"]) == 6
         if self.spectral:
             # particle::initParticles -> initializeSpectra (include/particle.h:241-242,
@@ -401,7 +419,7 @@ class Ensemble:
             self._stop_at = np.ascontiguousarray(stop_at, dtype=np.float64)
             sp = _ptr(self._stop_at)
         if self.spectral:
-            lib().or_run_simulation_spec(_ptr(self.opt), C.addressof(self._ext), C.addressof(self._a),
+            self._lib.or_run_simulation_spec(_ptr(self.opt), C.addressof(self._ext), C.addressof(self._a),
                                          self.n, next_t, sp, prev_t, _ptr(self.n_ode), nthreads,
                                          _ptr(self.agg), C.addressof(self.cfg))
             self.samples_total += self.agg["n_samples"].astype(np.uint64)
@@ -409,7 +427,7 @@ class Ensemble:
             self.cspec = self.cspec + tot
             self.cspec_n += tn
             return
-        lib().or_run_simulation(_ptr(self.opt), C.addressof(self._ext), C.addressof(self._a),
+        self._lib.or_run_simulation(_ptr(self.opt), C.addressof(self._ext), C.addressof(self._a),
                                 self.n, next_t, sp, prev_t, _ptr(self.n_ode), nthreads)
 
     def floquet_result(self) -> np.ndarray:

@@ -10,6 +10,9 @@
  * (a + (b*c)) + d, `s * v * w` is ((s*v)*w), and so on.
  */
 #include "st_oracle.h"
+/* the field hooks pulse()/grad() (src/integrator.cpp:40-46), shared with the
+ * HIP kernels so both evaluate the same, possibly user-edited, fields */
+#include "../include/spintrack_fields.h"
 
 #include <float.h>
 #include <math.h>
@@ -431,18 +434,28 @@ void or_interpolate(double t, double t0, or_coords p_old, or_coords v_old,
 
 /* src/integrator.cpp:40-42 (pulse), :44-46 (grad), :63-70 (findCrossTerm):
  * gamma * (pulse + B0 + (1/c^2) v x E + G.r) */
+/* grad(pos, opt) through the hook: {dot(p, Gx), dot(p, Gy), dot(p, Gz)} for
+ * the default hook (src/integrator.cpp:44-46) */
+static or_coords field_grad(or_coords p, const or_options *opt) {
+    const double r[3] = {p.x, p.y, p.z};
+    const double G[9] = {opt->Gx.x, opt->Gx.y, opt->Gx.z, opt->Gy.x, opt->Gy.y, opt->Gy.z,
+                         opt->Gz.x, opt->Gz.y, opt->Gz.z};
+    double B[3];
+    st_field_grad(r, G, B);
+    or_coords g = {B[0], B[1], B[2]};
+    return g;
+}
+
 or_coords or_cross_term(double t, const or_options *opt, const or_ext *ext,
         double t0, or_coords p_old, or_coords v_old) {
     or_coords p, v;
     or_interpolate(t, t0, p_old, v_old, &p, &v, opt);
-    or_coords G;
-    G.x = dot3(p, opt->Gx);
-    G.y = dot3(p, opt->Gy);
-    G.z = dot3(p, opt->Gz);
-    or_coords pl = {0.0, 0.0, 0.0};
-    const double pv = opt->a * cos(opt->w * t);
-    const int ax = ext ? ext->pulse_axis : 2;
-    if (ax == 0) pl.x = pv; else if (ax == 1) pl.y = pv; else pl.z = pv;
+    const or_coords G = field_grad(p, opt);
+    /* pulse(t, a, w), evaluated even when a == 0 as the reference does
+     * (src/integrator.cpp:67) */
+    double P3[3];
+    st_field_pulse(t, opt->a, opt->w, ext ? ext->pulse_axis : 2, P3);
+    const or_coords pl = {P3[0], P3[1], P3[2]};
     const or_coords vxE = cross3(v, opt->E);
     const double ic2 = 1.0 / C2;
     or_coords B;
@@ -1105,10 +1118,7 @@ or_coords or_noise_term(double t, const or_options *opt, double t0, or_coords p_
         or_coords v_old) {
     or_coords p, v;
     or_interpolate(t, t0, p_old, v_old, &p, &v, opt);
-    or_coords G;
-    G.x = dot3(p, opt->Gx);
-    G.y = dot3(p, opt->Gy);
-    G.z = dot3(p, opt->Gz);
+    const or_coords G = field_grad(p, opt);
     or_coords N;
     N.x = opt->noiseAmplitudes.x * sin(opt->noiseFrequencies.x * t);
     N.y = opt->noiseAmplitudes.y * sin(opt->noiseFrequencies.y * t);

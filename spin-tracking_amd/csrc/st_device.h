@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/spintrack_fields.h"
+
 // Device math is also compiled for the host so a CPU build of the same
 // functions can be checked against the oracle without a GPU.
 #define ST_HD __host__ __device__
@@ -457,8 +459,9 @@ struct Seg {
     d3 p0, v0;     // position and velocity at t0 (pos_old, v_old)
 };
 
-// Exact: interpolate -> grad -> pulse + B0 + (1/c^2) v x E + G.r -> gamma*B
-// (findCrossTerm, src/integrator.cpp:63-70), in the reference's order.
+// Exact: interpolate -> grad -> pulse + B0 + (1/c^2) v x E + grad -> gamma*B
+// (findCrossTerm, src/integrator.cpp:63-70), in the reference's order, with
+// the field hooks of spintrack_fields.h.
 ST_HD __forceinline__ d3 omega_exact(const SimParams& P, const Phys& F, double t, const Seg& S) {
     const double d = t - S.t0;
     d3 p, v;
@@ -475,22 +478,19 @@ ST_HD __forceinline__ d3 omega_exact(const SimParams& P, const Phys& F, double t
         p.z = S.p0.z + S.v0.z * d;
         v = S.v0;
     }
-    const double gx = p.x * F.G[0] + p.y * F.G[1] + p.z * F.G[2];
-    const double gy = p.x * F.G[3] + p.y * F.G[4] + p.z * F.G[5];
-    const double gz = p.x * F.G[6] + p.y * F.G[7] + p.z * F.G[8];
-    double plx = 0.0, ply = 0.0, plz = 0.0;
-    if (F.pulse_on) {
-        const double pv = F.pa * cos(F.pw * t);
-        if (F.pulse_axis == 0) plx = pv;
-        else if (F.pulse_axis == 1) ply = pv;
-        else plz = pv;
-    }
+    const double r[3] = {p.x, p.y, p.z};
+    double g[3];
+    st_field_grad(r, F.G, g);
+    double pl[3] = {0.0, 0.0, 0.0};
+    // a cos-axis pulse with a == 0 adds a signed zero, skipped bit-exactly
+    // (pulse_on is set on the host when it could matter)
+    if (!ST_FIELDS_PULSE_COS_AXIS || F.pulse_on) st_field_pulse(t, F.pa, F.pw, F.pulse_axis, pl);
     const double cx = v.y * F.Ez - v.z * F.Ey;
     const double cy = v.z * F.Ex - v.x * F.Ez;
     const double cz = v.x * F.Ey - v.y * F.Ex;
-    const double Bx = ((plx + F.B0x) + cx * IC2) + gx;
-    const double By = ((ply + F.B0y) + cy * IC2) + gy;
-    const double Bz = ((plz + F.B0z) + cz * IC2) + gz;
+    const double Bx = ((pl[0] + F.B0x) + cx * IC2) + g[0];
+    const double By = ((pl[1] + F.B0y) + cy * IC2) + g[1];
+    const double Bz = ((pl[2] + F.B0z) + cz * IC2) + g[2];
     return d3{Bx * F.gamma, By * F.gamma, Bz * F.gamma};
 }
 

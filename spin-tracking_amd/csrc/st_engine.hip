@@ -300,7 +300,8 @@ static Phys derive_phys(const st_options& o, const st_ext& x) {
     const double b0ax = x.pulse_axis == 0 ? o.B0.x : (x.pulse_axis == 1 ? o.B0.y : o.B0.z);
     // a*cos(w*t) with a == 0 is a signed zero; added to B0[axis] it changes
     // nothing unless B0[axis] is itself -0.0 (or cos is not finite).
-    F.pulse_on = (o.a != 0.0 || signbit(b0ax) || !isfinite(o.w)) ? 1 : 0;
+    // (a pulse hook of another form is evaluated always, as the reference's is)
+    F.pulse_on = (!ST_FIELDS_PULSE_COS_AXIS || o.a != 0.0 || signbit(b0ax) || !isfinite(o.w)) ? 1 : 0;
     F.tc = o.tc;
     F.sqrtKT_m = o.sqrtKT_m;
     F.diffuse = o.diffuse;
@@ -777,7 +778,22 @@ static floquet::Diag read_floquet(const st_floquet* f) {
 
 extern "C" {
 
-ST_API const char* st_version(void) { return "spintrack gfx950 exact,fast dop853,rk45,magnus,rk45q,rkf45,rkf45q,spectral pipeline,fused"; }
+#define ST_STR2(x) #x
+#define ST_STR(x) ST_STR2(x)
+#if ST_FIELDS_POLYNOMIAL
+#define ST_VERSION_FAST ",fast"
+#else
+#define ST_VERSION_FAST ""
+#endif
+#ifdef ST_FIELDS_FILE
+#define ST_VERSION_FIELDS ST_STR(ST_FIELDS_FILE)
+#else
+#define ST_VERSION_FIELDS "default"
+#endif
+ST_API const char* st_version(void) {
+    return "spintrack gfx950 exact" ST_VERSION_FAST " dop853,rk45,magnus,rk45q,rkf45,rkf45q,spectral pipeline,fused "
+           "fields=" ST_VERSION_FIELDS;
+}
 
 ST_API const char* st_last_error(const st_handle* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
 
@@ -788,6 +804,11 @@ static int check_ext(const st_ext* ext, st_ext& x) {
     if (x.pulse_axis < 0 || x.pulse_axis > 2) return fail(nullptr, ST_E_ARG, "pulse_axis must be 0, 1 or 2");
     if (x.variant != ST_VARIANT_EXACT && x.variant != ST_VARIANT_FAST)
         return fail(nullptr, ST_E_ARG, "unknown variant %d", x.variant);
+    if (!ST_FIELDS_POLYNOMIAL && x.variant == ST_VARIANT_FAST)
+        return fail(nullptr, ST_E_UNSUPPORTED,
+                    "the field hooks this library was built with (spintrack_fields.h) are not a linear gradient "
+                    "and a cos pulse, so the fast variant's per-segment field polynomial does not hold: use "
+                    "ST_VARIANT_EXACT");
     if (x.engine != ST_ENGINE_PIPELINE && x.engine != ST_ENGINE_FUSED)
         return fail(nullptr, ST_E_ARG, "unknown engine %d", x.engine);
     if (x.graph_chunks < 0 || x.graph_chunks > 64)
@@ -1105,6 +1126,9 @@ ST_API int st_set_checkpointing(st_handle* h, int enable) {
 
 ST_API int st_adjoint(st_handle* h, const void* blocks_dev, const double* seeds, double* grad) {
     if (!h || !blocks_dev || !seeds || !grad) return fail(h, ST_E_ARG, "null argument");
+    if (!ST_FIELDS_POLYNOMIAL)
+        return fail(h, ST_E_UNSUPPORTED, "st_adjoint differentiates the linear-gradient, cos-pulse field; this "
+                    "library was built with other field hooks (spintrack_fields.h)");
     if (h->ckpts.empty()) return fail(h, ST_E_STATE, "st_adjoint needs a checkpointed pipeline run first");
     const RunArgs& R0 = h->ckpt_run;
     const uint32_t K = (uint32_t)R0.n_targets;

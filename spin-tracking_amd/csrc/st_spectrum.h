@@ -82,9 +82,10 @@ __device__ __forceinline__ d3 noise_exact(const SimParams& P, const Phys& F, con
         p.z = G.p0.z + G.v0.z * d;
         v = G.v0;
     }
-    const double gx = p.x * F.G[0] + p.y * F.G[1] + p.z * F.G[2];
-    const double gy = p.x * F.G[3] + p.y * F.G[4] + p.z * F.G[5];
-    const double gz = p.x * F.G[6] + p.y * F.G[7] + p.z * F.G[8];
+    const double r[3] = {p.x, p.y, p.z};
+    double gg[3];
+    st_field_grad(r, F.G, gg);   // the grad hook of spintrack_fields.h
+    const double gx = gg[0], gy = gg[1], gz = gg[2];
     double nx = 0.0, ny = 0.0, nz = 0.0;
     if (S.noise_on) {
         nx = S.na[0] * sin(S.nw[0] * t);

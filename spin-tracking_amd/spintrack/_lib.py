@@ -182,31 +182,36 @@ TRACE_DOUBLES = 20
 TRACE_FIELDS = ["tag", "t_old", "t", "px", "py", "pz", "vx", "vy", "vz", "sx", "sy", "sz", "h",
                 "accepted", "rejected", "coll_type", "wall_hit", "status", "n_bounce", "n_coll"]
 
-_lib = None
+_libs = {}
 
 
 class SpinTrackError(RuntimeError):
     pass
 
 
-def load() -> C.CDLL:
-    """Loads libspintrack.so; raises if it has not been built."""
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
+def open_library(path: str) -> C.CDLL:
+    """Loads a build of the library (e.g. one compiled with other field hooks,
+    spin-tracking_amd/Makefile FIELDS=... LIBNAME=...); raises if missing."""
+    if path not in _libs:
+        if not os.path.exists(path):
             raise SpinTrackError(
-                f"{LIB_PATH} is missing: build it with `make -C spin-tracking_amd` "
+                f"{path} is missing: build it with `make -C spin-tracking_amd` "
                 "(or __graft_entry__.build()); there is no CPU fallback")
-        L = C.CDLL(LIB_PATH)
+        L = C.CDLL(path)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
-def check(rc: int, handle=None) -> None:
+def load() -> C.CDLL:
+    """Loads libspintrack.so; raises if it has not been built."""
+    return open_library(LIB_PATH)
+
+
+def check(rc: int, handle=None, lib: C.CDLL = None) -> None:
     if rc != ST_OK:
-        msg = load().st_last_error(handle)
+        msg = (lib or load()).st_last_error(handle)
         raise SpinTrackError(f"spintrack error {rc}: {msg.decode() if msg else ''}")

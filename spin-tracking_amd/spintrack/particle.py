@@ -56,14 +56,15 @@ class Particle:
 
     def __init__(self, opt: Options, pulse_axis: int = 2, variant: int = L.VARIANT_EXACT,
                  block_size: int = 0, engine: int = L.ENGINE_PIPELINE, seg_buffer_mb: int = 0,
-                 chunk_records: int = 0, graph_chunks: int = 0, spectrum_sum: int = 0, max_events: int = 0):
-        self.lib = L.load()
+                 chunk_records: int = 0, graph_chunks: int = 0, spectrum_sum: int = 0, max_events: int = 0,
+                 library: str = None):
+        self.lib = L.open_library(library) if library else L.load()
         self.opt = Options.from_buffer_copy(bytes(opt))
         self.n = int(opt.numParticles)
         self.ext = Ext(int(pulse_axis), int(variant), int(block_size), int(engine), int(seg_buffer_mb),
                        int(chunk_records), int(graph_chunks), int(spectrum_sum), int(max_events))
         h = C.c_void_p()
-        check(self.lib.st_create(C.byref(self.opt), C.byref(self.ext), C.byref(h)), None)
+        check(self.lib.st_create(C.byref(self.opt), C.byref(self.ext), C.byref(h)), None, self.lib)
         self.h = h
 
     @classmethod
@@ -82,16 +83,16 @@ class Particle:
         self.ext = Ext(int(pulse_axis), int(variant), int(block_size), L.ENGINE_PIPELINE, int(seg_buffer_mb),
                        int(chunk_records), int(graph_chunks))
         h = C.c_void_p()
-        check(self.lib.st_create_batch(arr, len(cfgs), C.byref(self.ext), C.byref(h)), None)
+        check(self.lib.st_create_batch(arr, len(cfgs), C.byref(self.ext), C.byref(h)), None, self.lib)
         self.h = h
         return self
 
     def layout(self) -> np.ndarray:
         """Particle offsets of the configurations (n_cfg + 1 entries)."""
         n = C.c_int32()
-        check(self.lib.st_batch_layout(self.h, C.byref(n), None), self.h)
+        check(self.lib.st_batch_layout(self.h, C.byref(n), None), self.h, self.lib)
         off = np.zeros(n.value + 1, np.uint32)
-        check(self.lib.st_batch_layout(self.h, None, _ptr(off)), self.h)
+        check(self.lib.st_batch_layout(self.h, None, _ptr(off)), self.h, self.lib)
         return off
 
     # -- lifetime -----------------------------------------------------------
@@ -115,11 +116,11 @@ class Particle:
     # -- the reference's interface -----------------------------------------
     def initParticles(self, first: int = 0, stride: int = 1) -> None:
         """include/particle.h:241-252 / src/particle.cpp:638-710."""
-        check(self.lib.st_init_subset(self.h, first, stride), self.h)
+        check(self.lib.st_init_subset(self.h, first, stride), self.h, self.lib)
 
     def runSimulation(self, next_t: float) -> None:
         """src/particle.cpp:865-876: advance every particle to next_t."""
-        check(self.lib.st_advance_to(self.h, float(next_t)), self.h)
+        check(self.lib.st_advance_to(self.h, float(next_t)), self.h, self.lib)
 
     def snapshot(self) -> dict:
         """The eight arrays of one snapshot, in the reference's types."""
@@ -128,7 +129,7 @@ class Particle:
                    S=np.empty(n, COORDS), failure=np.empty(n, np.int32),
                    n_coll=np.empty(n, np.uint64), n_bounce=np.empty(n, np.uint64),
                    n_steps=np.empty(n, np.uint64))
-        check(self.lib.st_snapshot(self.h, *[_ptr(out[k]) for k in SNAPSHOT_FIELDS]), self.h)
+        check(self.lib.st_snapshot(self.h, *[_ptr(out[k]) for k in SNAPSHOT_FIELDS]), self.h, self.lib)
         return out
 
     def outputData(self, logger) -> None:
@@ -140,46 +141,46 @@ class Particle:
     def run(self, targets, blocks_dev_ptr: int = 0) -> None:
         """Persistent multi-interval run (st_run)."""
         t = np.ascontiguousarray(targets, dtype=np.float64)
-        check(self.lib.st_run(self.h, len(t), _ptr(t), blocks_dev_ptr or None), self.h)
+        check(self.lib.st_run(self.h, len(t), _ptr(t), blocks_dev_ptr or None), self.h, self.lib)
 
     def set_stop_times(self, stop_at) -> None:
         if stop_at is None:
-            check(self.lib.st_set_stop_times(self.h, None), self.h)
+            check(self.lib.st_set_stop_times(self.h, None), self.h, self.lib)
             return
         s = np.ascontiguousarray(stop_at, dtype=np.float64)
         assert s.shape == (self.n,)
-        check(self.lib.st_set_stop_times(self.h, _ptr(s)), self.h)
+        check(self.lib.st_set_stop_times(self.h, _ptr(s)), self.h, self.lib)
 
     def block_bytes(self) -> int:
         return int(self.lib.st_snapshot_block_bytes(self.h))
 
     def snapshot_block_device(self, dst_ptr: int) -> None:
-        check(self.lib.st_snapshot_block_device(self.h, dst_ptr), self.h)
+        check(self.lib.st_snapshot_block_device(self.h, dst_ptr), self.h, self.lib)
 
     def get_state(self) -> np.ndarray:
         out = np.zeros(self.n, STATE_DTYPE)
-        check(self.lib.st_get_state(self.h, _ptr(out)), self.h)
+        check(self.lib.st_get_state(self.h, _ptr(out)), self.h, self.lib)
         return out
 
     def set_state(self, state: np.ndarray) -> None:
         s = np.ascontiguousarray(state, dtype=STATE_DTYPE)
         assert s.shape == (self.n,)
-        check(self.lib.st_set_state(self.h, _ptr(s)), self.h)
+        check(self.lib.st_set_state(self.h, _ptr(s)), self.h, self.lib)
 
     def work_counters(self) -> dict:
         w = np.zeros(6, np.uint64)
-        check(self.lib.st_work_counters(self.h, _ptr(w)), self.h)
+        check(self.lib.st_work_counters(self.h, _ptr(w)), self.h, self.lib)
         return dict(accepted=int(w[0]), rejected=int(w[1]), segments=int(w[2]), trips=int(w[3]),
                     events=int(w[4]), kin_records=int(w[5]))
 
     def particle_counters(self) -> dict:
         a, r, s = (np.zeros(self.n, np.uint64) for _ in range(3))
-        check(self.lib.st_particle_counters(self.h, _ptr(a), _ptr(r), _ptr(s)), self.h)
+        check(self.lib.st_particle_counters(self.h, _ptr(a), _ptr(r), _ptr(s)), self.h, self.lib)
         return dict(accepted=a, rejected=r, segments=s)
 
     def set_trace(self, particle: int, capacity: int = 100000) -> None:
         """Record every integrated segment of one particle (debug tracing)."""
-        check(self.lib.st_set_trace(self.h, int(particle), int(capacity)), self.h)
+        check(self.lib.st_set_trace(self.h, int(particle), int(capacity)), self.h, self.lib)
         self._trace_cap = int(capacity)
 
     def get_trace(self) -> np.ndarray:
@@ -187,16 +188,16 @@ class Particle:
         cap = getattr(self, "_trace_cap", 0)
         out = np.zeros((max(cap, 1), L.TRACE_DOUBLES))
         n = C.c_uint32()
-        check(self.lib.st_get_trace(self.h, _ptr(out), cap, C.byref(n)), self.h)
+        check(self.lib.st_get_trace(self.h, _ptr(out), cap, C.byref(n)), self.h, self.lib)
         return out[: n.value]
 
     def copy_to_host(self, dev_ptr: int, nbytes: int) -> np.ndarray:
         out = np.empty(int(nbytes), np.uint8)
-        check(self.lib.st_memcpy_d2h(self.h, _ptr(out), C.c_void_p(int(dev_ptr)), int(nbytes)), self.h)
+        check(self.lib.st_memcpy_d2h(self.h, _ptr(out), C.c_void_p(int(dev_ptr)), int(nbytes)), self.h, self.lib)
         return out
 
     def set_checkpointing(self, enable: bool = True) -> None:
-        check(self.lib.st_set_checkpointing(self.h, 1 if enable else 0), self.h)
+        check(self.lib.st_set_checkpointing(self.h, 1 if enable else 0), self.h, self.lib)
 
     def adjoint(self, blocks_dev_ptr: int, seeds) -> np.ndarray:
         """Gradient (n_cfg, 15) of a loss of the ensemble-mean spins, given
@@ -204,7 +205,7 @@ class Particle:
         n_cfg = len(self.layout()) - 1
         s = np.ascontiguousarray(seeds, dtype=np.float64).reshape(n_cfg, -1, 3)
         out = np.zeros((n_cfg, 15))
-        check(self.lib.st_adjoint(self.h, C.c_void_p(int(blocks_dev_ptr)), _ptr(s), _ptr(out)), self.h)
+        check(self.lib.st_adjoint(self.h, C.c_void_p(int(blocks_dev_ptr)), _ptr(s), _ptr(out)), self.h, self.lib)
         return out
 
     def block_sums(self, blocks_dev_ptr: int, n_blocks: int) -> np.ndarray:
@@ -212,7 +213,7 @@ class Particle:
         snapshot block and configuration."""
         n_cfg = len(self.layout()) - 1
         out = np.zeros((int(n_blocks), n_cfg, 4))
-        check(self.lib.st_block_sums(self.h, C.c_void_p(int(blocks_dev_ptr)), int(n_blocks), _ptr(out)), self.h)
+        check(self.lib.st_block_sums(self.h, C.c_void_p(int(blocks_dev_ptr)), int(n_blocks), _ptr(out)), self.h, self.lib)
         return out
 
     def phase_stats(self, blocks_dev_ptr: int, n_blocks: int, phi_dev_ptr: int = 0) -> np.ndarray:
@@ -223,43 +224,43 @@ class Particle:
         n_cfg = len(self.layout()) - 1
         out = np.zeros((int(n_blocks), n_cfg, 6))
         check(self.lib.st_phase_stats(self.h, C.c_void_p(int(blocks_dev_ptr)), int(n_blocks), _ptr(out),
-                                      C.c_void_p(int(phi_dev_ptr)) if phi_dev_ptr else None), self.h)
+                                      C.c_void_p(int(phi_dev_ptr)) if phi_dev_ptr else None), self.h, self.lib)
         return out
 
     def copy_device(self, dst_ptr: int, src_ptr: int, nbytes: int) -> None:
         check(self.lib.st_memcpy_d2d(self.h, C.c_void_p(int(dst_ptr)), C.c_void_p(int(src_ptr)),
-                                     int(nbytes)), self.h)
+                                     int(nbytes)), self.h, self.lib)
 
     def kernel_timing(self, enable: bool = True) -> None:
-        check(self.lib.st_kernel_timing(self.h, 1 if enable else 0), self.h)
+        check(self.lib.st_kernel_timing(self.h, 1 if enable else 0), self.h, self.lib)
 
     def kernel_times(self) -> dict:
         ms = np.zeros(2)
         n = np.zeros(2, np.uint64)
-        check(self.lib.st_kernel_times(self.h, _ptr(ms), _ptr(n)), self.h)
+        check(self.lib.st_kernel_times(self.h, _ptr(ms), _ptr(n)), self.h, self.lib)
         return dict(kin_ms=float(ms[0]), spin_ms=float(ms[1]), kin_launches=int(n[0]),
                     spin_launches=int(n[1]))
 
     def timer_start(self) -> None:
-        check(self.lib.st_timer_start(self.h), self.h)
+        check(self.lib.st_timer_start(self.h), self.h, self.lib)
 
     def timer_stop(self) -> float:
         """Milliseconds since timer_start, measured with events on this stream."""
         ms = C.c_float()
-        check(self.lib.st_timer_stop(self.h, C.byref(ms)), self.h)
+        check(self.lib.st_timer_stop(self.h, C.byref(ms)), self.h, self.lib)
         return float(ms.value)
 
     def set_stream(self, stream_ptr: int) -> None:
-        check(self.lib.st_set_stream(self.h, stream_ptr or None), self.h)
+        check(self.lib.st_set_stream(self.h, stream_ptr or None), self.h, self.lib)
 
     def synchronize(self) -> None:
-        check(self.lib.st_synchronize(self.h), self.h)
+        check(self.lib.st_synchronize(self.h), self.h, self.lib)
 
     def spinMean(self) -> np.ndarray:
         """particle::spinMean (src/particle.cpp:878-884): the mean spin over
         all particles (a stopped particle's NaN spin makes it NaN)."""
         S = np.zeros(self.n, dtype=COORDS)
-        check(self.lib.st_snapshot(self.h, None, None, None, _ptr(S), None, None, None, None), self.h)
+        check(self.lib.st_snapshot(self.h, None, None, None, _ptr(S), None, None, None, None), self.h, self.lib)
         # the reference's order, S_sum = S_sum + S[i] (cumsum adds in sequence)
         return np.cumsum(xyz(S), axis=0)[-1] / self.n
 
@@ -267,7 +268,7 @@ class Particle:
     def spectrum(self) -> dict:
         """particle::cspec: the summed, un-normalised covariance spectrum."""
         sp = L.Spectrum()
-        check(self.lib.st_spectrum_get(self.h, C.byref(sp)), self.h)
+        check(self.lib.st_spectrum_get(self.h, C.byref(sp)), self.h, self.lib)
         v = np.ctypeslib.as_array(sp.variance).reshape(L.NW, 3, 3, 2)
         return dict(frequencies=np.array(sp.frequencies[:]), dt=sp.dt, n_samples=int(sp.n_samples),
                     variance=v[..., 0] + 1j * v[..., 1], raw=v.copy())
@@ -277,21 +278,21 @@ class Particle:
         s1, s2, imag_diag (n, NW, 3) and sample counts."""
         agg = np.zeros((self.n, L.AGG_FIELDS))
         ns = np.zeros(self.n, dtype=np.uint32)
-        check(self.lib.st_spectrum_aggregators(self.h, _ptr(agg), _ptr(ns)), self.h)
+        check(self.lib.st_spectrum_aggregators(self.h, _ptr(agg), _ptr(ns)), self.h, self.lib)
         a = agg.reshape(self.n, 3, L.NW, 3)
         return dict(s1=a[:, 0].copy(), s2=a[:, 1].copy(), im=a[:, 2].copy(), n_samples=ns)
 
     def floquetInfo(self) -> dict:
         """particle::fd (floquet_diagonalize of the run's drive)."""
         f = L.Floquet()
-        check(self.lib.st_floquet_info(self.h, C.byref(f)), self.h)
+        check(self.lib.st_floquet_info(self.h, C.byref(f)), self.h, self.lib)
         from .floquet import floquet_dict
         return floquet_dict(f)
 
     def floquetResults(self) -> np.ndarray:
         """particle::floquetResults (src/particle.cpp:841-843): b_end."""
         b = np.zeros(3)
-        check(self.lib.st_floquet_result(self.h, _ptr(b)), self.h)
+        check(self.lib.st_floquet_result(self.h, _ptr(b)), self.h, self.lib)
         return b
 
     def postProcess(self, logger) -> None:
