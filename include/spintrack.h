@@ -202,6 +202,21 @@ ST_API int st_snapshot_block_device(st_handle* h, void* dst);
 ST_API int st_run(st_handle* h, int32_t n_targets, const double* targets,
                   void* blocks_dev);
 
+/* No counterpart in the reference: the form of st_run for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and per-particle results
+ * as st_run, but it calls no hipMalloc/hipFree: the output times are staged
+ * and the kernels' work-queue and list counters kept in one static device
+ * array of the library (ST_NOALLOC_MAX_TARGETS * 8 + 256 bytes per GPU, held
+ * until the process ends), so n_targets is at most ST_NOALLOC_MAX_TARGETS.
+ * The handle must be seeded (st_init allocates the ensemble and the
+ * pipeline's record buffer) and must not have checkpointing on; a handle
+ * with graph_chunks runs the host-driven chunk loop here.  Every call on one
+ * GPU shares the array and the call waits for its work before returning:
+ * calls on one GPU from several host threads at once race. */
+#define ST_NOALLOC_MAX_TARGETS 4096
+ST_API int st_run_noalloc(st_handle* h, int32_t n_targets, const double* targets, void* blocks_dev);
+
 /* Per-particle end times (batched extension; the reference overwrites its
  * per-particle tf array, src/particle.cpp:748-749).  Particle i takes part
  * in an interval (t_prev, t_next] only while t_prev < stop_at[i], and then

@@ -23,6 +23,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -208,6 +209,7 @@ struct st_handle {
     bool have_stop_at = false;
     unsigned long long* d_work = nullptr;
     uint32_t* d_small = nullptr;   // [0] queue, [1] next_count, [2] fail_count
+    int device = -1;               // the device current at st_create
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     bool initialized = false;
@@ -860,6 +862,7 @@ static int create_common(const st_options& opt, const st_ext& x, uint32_t n, st_
     h->P = derive_params(opt, x);
     h->any_pulse = h->P.ph.pulse_on != 0;
     h->state_bytes = (size_t)field_stride(h->n) * F_COUNT;
+    if (hipGetDevice(&h->device) != hipSuccess) h->device = -1;
     hipError_t e = hipMalloc(&h->state, h->state_bytes);
     if (e == hipSuccess) e = hipMalloc(&h->d_work, N_WORK * sizeof(unsigned long long));
     if (e == hipSuccess) e = hipMalloc(&h->d_small, 256);
@@ -1111,6 +1114,81 @@ ST_API int st_run(st_handle* h, int32_t n_targets, const double* targets, void* 
     if (r) return r;
     h->last_target = targets[n_targets - 1];
     return ST_OK;
+}
+
+// ---- st_run_noalloc: st_run with its scratch in a static device array ------
+// The staging that st_run allocates (the device copy of the output times) and
+// the counters its kernels share (work-queue heads, list lengths, failure
+// count; the handle's 256-byte counter block otherwise) live in one
+// file-scope device array.  Host code never passes `scratch` itself to a
+// kernel or a copy: it takes the array's device address per device with
+// hipGetSymbolAddress, once, on the first call on that device.
+constexpr size_t NOALLOC_COUNTERS = 256;
+constexpr size_t NOALLOC_BYTES = NOALLOC_COUNTERS + sizeof(double) * ST_NOALLOC_MAX_TARGETS;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_addr[NOALLOC_MAX_DEVICES] = {};
+
+static int scratch_address(st_handle* h, void** out) {
+    int dev = -1;
+    HIPCHK(h, hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(h, ST_E_ARG, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(h, ST_E_HIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+        g_scratch_addr[dev] = ptr;
+    }
+    *out = g_scratch_addr[dev];
+    return ST_OK;
+}
+
+ST_API int st_run_noalloc(st_handle* h, int32_t n_targets, const double* targets, void* blocks_dev) {
+    if (!h) return fail(nullptr, ST_E_ARG, "null handle");
+    if (!h->initialized) return fail(h, ST_E_STATE, "st_run_noalloc before st_init");
+    if (n_targets <= 0 || !targets) return fail(h, ST_E_ARG, "st_run_noalloc needs at least one target");
+    if (n_targets > ST_NOALLOC_MAX_TARGETS)
+        return fail(h, ST_E_ARG, "st_run_noalloc takes at most %d output times (got %d); use st_run",
+                    ST_NOALLOC_MAX_TARGETS, n_targets);
+    if (h->ckpt_on)
+        return fail(h, ST_E_UNSUPPORTED, "st_run_noalloc does not checkpoint (checkpoints are allocated per run)");
+    if (h->device != -1) {
+        int dev = -1;
+        HIPCHK(h, hipGetDevice(&dev));
+        if (dev != h->device)
+            return fail(h, ST_E_STATE, "st_run_noalloc: the handle lives on device %d, device %d is current", h->device,
+                        dev);
+    }
+    void* base = nullptr;
+    int r = scratch_address(h, &base);
+    if (r) return r;
+    // the run uses the static array for its counters and staging; the
+    // host-driven chunk loop (a captured graph would be instantiated, which
+    // allocates) gives the same results as the graph replay
+    uint32_t* const own_small = h->d_small;
+    double* const own_targets = h->d_targets;
+    const int32_t own_cap = h->targets_cap;
+    const int32_t own_graph = h->ext.graph_chunks;
+    h->d_small = (uint32_t*)base;
+    h->d_targets = (double*)((char*)base + NOALLOC_COUNTERS);
+    h->targets_cap = ST_NOALLOC_MAX_TARGETS;
+    h->ext.graph_chunks = 0;
+    // the counters start at zero for this call (the array keeps the last call's values)
+    hipError_t e = hipMemsetAsync(base, 0, NOALLOC_COUNTERS, h->stream);
+    r = e == hipSuccess ? st_run(h, n_targets, targets, blocks_dev)
+                        : fail(h, ST_E_HIP, "hipMemsetAsync failed: %s", hipGetErrorString(e));
+    // the array is shared by every call on this device: finish before returning
+    e = hipStreamSynchronize(h->stream);
+    h->d_small = own_small;
+    h->d_targets = own_targets;
+    h->targets_cap = own_cap;
+    h->ext.graph_chunks = own_graph;
+    if (!r && e != hipSuccess) r = fail(h, ST_E_HIP, "st_run_noalloc: %s", hipGetErrorString(e));
+    return r;
 }
 
 ST_API int st_set_checkpointing(st_handle* h, int enable) {

@@ -20,6 +20,7 @@ ST_E_STATE = -4
 FAIL_WATCHDOG = -9            # failureState of a particle stopped by the event watchdog
 DEFAULT_MAX_EVENTS = 100000000
 MAX_PARTICLES = 1 << 29       # particles per handle (32-bit lane offsets)
+NOALLOC_MAX_TARGETS = 4096    # output times per st_run_noalloc call
 
 VARIANT_EXACT = 0
 VARIANT_FAST = 1
@@ -121,6 +122,7 @@ SIGNATURES = {
     "st_snapshot_block_bytes": (_sz, [_vp]),
     "st_snapshot_block_device": (C.c_int, [_vp, _vp]),
     "st_run": (C.c_int, [_vp, _i32, _vp, _vp]),
+    "st_run_noalloc": (C.c_int, [_vp, _i32, _vp, _vp]),
     "st_set_stop_times": (C.c_int, [_vp, _vp]),
     "st_get_state": (C.c_int, [_vp, _vp]),
     "st_set_state": (C.c_int, [_vp, _vp]),

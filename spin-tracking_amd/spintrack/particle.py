@@ -143,6 +143,12 @@ class Particle:
         t = np.ascontiguousarray(targets, dtype=np.float64)
         check(self.lib.st_run(self.h, len(t), _ptr(t), blocks_dev_ptr or None), self.h, self.lib)
 
+    def run_noalloc(self, targets, blocks_dev_ptr: int = 0) -> None:
+        """st_run without device allocation (static scratch; at most
+        NOALLOC_MAX_TARGETS output times)."""
+        t = np.ascontiguousarray(targets, dtype=np.float64)
+        check(self.lib.st_run_noalloc(self.h, len(t), _ptr(t), blocks_dev_ptr or None), self.h, self.lib)
+
     def set_stop_times(self, stop_at) -> None:
         if stop_at is None:
             check(self.lib.st_set_stop_times(self.h, None), self.h, self.lib)
