@@ -86,8 +86,8 @@ typedef struct st_ext {
     int32_t block_size;     /* threads per workgroup of the kinematics kernel (pipeline) or of the
                                fused per-interval kernel; multiples of 64 up to 256; 0 = 256 */
     int32_t engine;         /* ST_ENGINE_* */
-    int32_t seg_buffer_mb;  /* HBM budget of the pipeline's segment-record buffer; 0 = 65536, at
-                               most 40% of the device memory free when the ensemble is seeded */
+    int32_t seg_buffer_mb;  /* HBM budget of the pipeline's segment-record buffer; 0 = 160 GiB, at
+                               most half the device memory free when the ensemble is seeded */
     int32_t chunk_records;  /* trajectory segments per particle per pipeline chunk (and between
                                adjoint checkpoints); 0 = as many as seg_buffer_mb holds */
     int32_t graph_chunks;   /* pipeline engine: capture this many chunks (rounded up to even, at

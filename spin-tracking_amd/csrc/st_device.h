@@ -229,8 +229,13 @@ struct Kin {
 
 // src/particle.cpp:121-257 -- closed-form wall hit times (quadratic in y
 // under gravity), the else-if fix-up chain, tie rules, gas vs wall, and the
-// maxPosStep / interval-end cap.
-ST_HD inline void next_collision(const SimParams& P, const Phys& F, double tf, Kin& K, Rng& rng) {
+// maxPosStep / interval-end cap.  A gas collision's draw of the next gas
+// time (:228) is left to the caller (gas_draw = true): the next draw from the
+// particle's stream is the re-emission's, so drawing the gas time just before
+// it keeps the stream order, and the two draws of a gas event sit together in
+// gas_job below.  next_collision() is the whole function.
+ST_HD inline void next_collision_core(const SimParams& P, const Phys& F, double tf, Kin& K, Rng& rng,
+                                      bool& gas_draw) {
     const d3 v = K.v, pos = K.pos;
     const double t = K.t;
     double dx, dy, dz, dtx, dty, dtz;
@@ -341,7 +346,7 @@ ST_HD inline void next_collision(const SimParams& P, const Phys& F, double tf, K
         if (gas_next) {
             K.coll_type = 'G';
             K.n_coll += 1;
-            K.next_gas += expo_draw(rng, F.tc);
+            gas_draw = true;   // K.next_gas += expo_draw(rng, F.tc), in gas_job
         } else {
             K.n_bounce += 1;
             K.coll_type = 'W';
@@ -354,6 +359,12 @@ ST_HD inline void next_collision(const SimParams& P, const Phys& F, double tf, K
     }
 }
 
+ST_HD inline void next_collision(const SimParams& P, const Phys& F, double tf, Kin& K, Rng& rng) {
+    bool gas_draw = false;
+    next_collision_core(P, F, tf, K, rng, gas_draw);
+    if (gas_draw) K.next_gas += expo_draw(rng, F.tc);
+}
+
 ST_HD __forceinline__ void flip_v(d3& v, char wh) {
     // src/particle.cpp:329-371
     if (wh == 'x' || wh == 'a' || wh == 'b' || wh == 'c') v.x *= -1.0;
@@ -361,8 +372,70 @@ ST_HD __forceinline__ void flip_v(d3& v, char wh) {
     if (wh == 'z' || wh == 'a' || wh == 'c' || wh == 'd') v.z *= -1.0;
 }
 
-// src/particle.cpp:263-388
-ST_HD inline void move_and_collide(const SimParams& P, const Phys& F, Kin& K, Rng& rng) {
+// The re-emissions of update_position_and_velocity, the expensive parts of
+// an event, as functions of their own: move_core() names the one an event
+// needs and the caller runs it (move_and_collide() below, the kinematics
+// kernel's kin_step).  Running them elsewhere than in line changes nothing:
+// same operations, same operands, same stream order.
+enum { JOB_NONE = 0, JOB_GAS = 1, JOB_DIFFUSE = 2 };
+
+// src/particle.cpp:306-328: Lambertian re-emission from wall wh at the
+// post-flight speed Vel (v: the post-flight velocity, for the signs); the
+// reference draws phi and theta before it looks at the wall id, and stops the
+// particle on an edge or corner.
+ST_HD inline void diffuse_job(d3& v, bool& stop, char wh, double Vel, Rng& rng) {
+    const double phi = acos(sqrt(uni(rng)));
+    const double theta = unif02pi(rng);
+    // sincos shares one argument reduction between the two values
+    // (each equals the separate sin / cos of the device libm)
+    double sp, cp, sth, cth;
+    sincos(phi, &sp, &cp);
+    sincos(theta, &sth, &cth);
+    if (wh == 'x') {
+        v.x = -1.0 * psgn(v.x) * Vel * cp;
+        v.y = -Vel * sp * cth;
+        v.z = Vel * sp * sth;
+    } else if (wh == 'y') {
+        v.x = Vel * sp * cth;
+        v.y = -1.0 * psgn(v.y) * Vel * cp;
+        v.z = Vel * sp * sth;
+    } else if (wh == 'z') {
+        v.x = Vel * sp * cth;
+        v.y = Vel * sp * sth;
+        v.z = -1.0 * psgn(v.z) * Vel * cp;
+    } else {
+        stop = true;
+    }
+}
+
+// A gas collision: the next gas time when it was deferred from
+// next_collision_core (draw_gas_time; src/particle.cpp:228), then the
+// re-emission (:373-387): 'M' gives three Maxwell-Boltzmann components, 'C'
+// an isotropic direction at the current speed Vel; both draw three normals
+// in the same order.
+ST_HD inline void gas_job(d3& v, double& next_gas, bool draw_gas_time, double Vel, double tc, double sqrtKT_m,
+                          int32_t dist, Rng& rng) {
+    if (draw_gas_time) next_gas += expo_draw(rng, tc);
+    if (dist == 'M' || dist == 'C') {
+        double n0, n1, n2;
+        normal3(rng, n0, n1, n2);
+        if (dist == 'M') {
+            v.x = n0 * sqrtKT_m;
+            v.y = n1 * sqrtKT_m;
+            v.z = n2 * sqrtKT_m;
+        } else {
+            const double vn = len3(d3{n0, n1, n2});
+            v.x = (Vel * n0) / vn;
+            v.y = (Vel * n1) / vn;
+            v.z = (Vel * n2) / vn;
+        }
+    }
+}
+
+// src/particle.cpp:263-388 up to the re-emission: the move, the wall snap and
+// the diffuse draw; returns the re-emission job the event needs and the
+// post-flight speed it uses.
+ST_HD inline int move_core(const SimParams& P, const Phys& F, Kin& K, Rng& rng, double& Vel) {
     K.v_old = K.v;
     K.t_old = K.t;
     K.t += K.dt;
@@ -377,7 +450,7 @@ ST_HD inline void move_and_collide(const SimParams& P, const Phys& F, Kin& K, Rn
     K.v.x = K.v.x + 0.0;
     K.v.y = K.v.y + ay * d;
     K.v.z = K.v.z + 0.0;
-    const double Vel = len3(K.v);
+    Vel = len3(K.v);
     const char ct = K.coll_type, wh = K.wall_hit;
     if (ct == 'W') {
         if (wh == 'x' || wh == 'a' || wh == 'b' || wh == 'c') K.pos.x = psgn(K.pos.x) * P.Lx * 0.5;
@@ -385,52 +458,20 @@ ST_HD inline void move_and_collide(const SimParams& P, const Phys& F, Kin& K, Rn
         if (wh == 'z' || wh == 'a' || wh == 'c' || wh == 'd') K.pos.z = psgn(K.pos.z) * P.Lz * 0.5;
         bool diffuse = false;
         if (F.diffuse_on) diffuse = uni(rng) < F.diffuse;
-        if (diffuse) {
-            // Lambertian re-emission at the post-flight speed; the reference
-            // draws phi and theta before it looks at the wall id.
-            const double phi = acos(sqrt(uni(rng)));
-            const double theta = unif02pi(rng);
-            // sincos shares one argument reduction between the two values
-            // (each equals the separate sin / cos of the device libm)
-            double sp, cp, sth, cth;
-            sincos(phi, &sp, &cp);
-            sincos(theta, &sth, &cth);
-            if (wh == 'x') {
-                K.v.x = -1.0 * psgn(K.v.x) * Vel * cp;
-                K.v.y = -Vel * sp * cth;
-                K.v.z = Vel * sp * sth;
-            } else if (wh == 'y') {
-                K.v.x = Vel * sp * cth;
-                K.v.y = -1.0 * psgn(K.v.y) * Vel * cp;
-                K.v.z = Vel * sp * sth;
-            } else if (wh == 'z') {
-                K.v.x = Vel * sp * cth;
-                K.v.y = Vel * sp * sth;
-                K.v.z = -1.0 * psgn(K.v.z) * Vel * cp;
-            } else {
-                K.stop = true;
-            }
-        } else {
-            flip_v(K.v, wh);
-        }
-    } else if (ct == 'G') {
-        // 'M': three Maxwell-Boltzmann components; 'C': isotropic direction at
-        // the current speed.  Both draw three normals in the same order.
-        if (F.dist == 'M' || F.dist == 'C') {
-            double n0, n1, n2;
-            normal3(rng, n0, n1, n2);
-            if (F.dist == 'M') {
-                K.v.x = n0 * F.sqrtKT_m;
-                K.v.y = n1 * F.sqrtKT_m;
-                K.v.z = n2 * F.sqrtKT_m;
-            } else {
-                const double vn = len3(d3{n0, n1, n2});
-                K.v.x = (Vel * n0) / vn;
-                K.v.y = (Vel * n1) / vn;
-                K.v.z = (Vel * n2) / vn;
-            }
-        }
+        if (diffuse) return JOB_DIFFUSE;
+        flip_v(K.v, wh);
+        return JOB_NONE;
     }
+    if (ct == 'G') return JOB_GAS;
+    return JOB_NONE;
+}
+
+// src/particle.cpp:263-388, after next_collision() (which drew the gas time)
+ST_HD inline void move_and_collide(const SimParams& P, const Phys& F, Kin& K, Rng& rng) {
+    double Vel;
+    const int job = move_core(P, F, K, rng, Vel);
+    if (job == JOB_DIFFUSE) diffuse_job(K.v, K.stop, K.wall_hit, Vel, rng);
+    else if (job == JOB_GAS) gas_job(K.v, K.next_gas, false, Vel, F.tc, F.sqrtKT_m, F.dist, rng);
 }
 
 // src/particle.cpp:412-444

@@ -337,16 +337,18 @@ static SimParams derive_params(const st_options& o, const st_ext& x) {
 static uint64_t field_stride(uint32_t n) { return ((uint64_t)8 * n + 255) & ~(uint64_t)255; }
 
 // ---- kernel selection ------------------------------------------------------
-template <int VAR, int INTEG, bool PERSIST>
+template <int VAR, int INTEG, bool PERSIST, bool PULSE = true>
 static hipError_t launch_fused(st_handle* h, dim3 grid, dim3 block, const RunArgs& R) {
-    hipLaunchKernelGGL((k_engine<VAR, INTEG, PERSIST>), grid, block, 0, h->stream, h->P, h->D, R);
+    hipLaunchKernelGGL((k_engine<VAR, INTEG, PERSIST, PULSE>), grid, block, 0, h->stream, h->P, h->D, R);
     return hipGetLastError();
 }
 using FusedFn = hipError_t (*)(st_handle*, dim3, dim3, const RunArgs&);
 template <bool PERSIST>
-static FusedFn pick_fused(int variant, int integ) {
+static FusedFn pick_fused(int variant, int integ, bool pulse) {
     if (variant == ST_VARIANT_FAST) {
-        if (integ == 0) return launch_fused<VAR_FAST, INT_DOP853, PERSIST>;
+        if (integ == 0)
+            return pulse ? launch_fused<VAR_FAST, INT_DOP853, PERSIST, true>
+                         : launch_fused<VAR_FAST, INT_DOP853, PERSIST, false>;
         if (integ == 1) return launch_fused<VAR_FAST, INT_RK45, PERSIST>;
         return launch_fused<VAR_FAST, INT_NONE, PERSIST>;
     }
@@ -354,9 +356,10 @@ static FusedFn pick_fused(int variant, int integ) {
     if (integ == 1) return launch_fused<VAR_EXACT, INT_RK45, PERSIST>;
     return launch_fused<VAR_EXACT, INT_NONE, PERSIST>;
 }
-static const void* fused_symbol(int variant, int integ) {
+static const void* fused_symbol(int variant, int integ, bool pulse) {
     if (variant == ST_VARIANT_FAST)
-        return integ == 0 ? (const void*)&k_engine<VAR_FAST, INT_DOP853, true>
+        return integ == 0 ? (pulse ? (const void*)&k_engine<VAR_FAST, INT_DOP853, true, true>
+                                   : (const void*)&k_engine<VAR_FAST, INT_DOP853, true, false>)
                           : (const void*)&k_engine<VAR_FAST, INT_RK45, true>;
     return integ == 0 ? (const void*)&k_engine<VAR_EXACT, INT_DOP853, true>
                       : (const void*)&k_engine<VAR_EXACT, INT_RK45, true>;
@@ -370,7 +373,7 @@ static const void* fused_symbol(int variant, int integ) {
 // fields make the spills cost more than the third wave gains.
 template <int VAR, int INTEG, bool BATCH, bool PULSE>
 constexpr bool use_o3() {
-    return VAR == VAR_FAST && INTEG == INT_DOP853 && !(BATCH && PULSE);
+    return VAR == VAR_FAST && INTEG == INT_DOP853;
 }
 template <int VAR, int INTEG, bool BATCH, bool PULSE>
 static hipError_t launch_spin(st_handle* h, dim3 grid, const PipeArgs& A) {
@@ -457,14 +460,18 @@ static RunArgs base_args(st_handle* h) {
 // ---- pipeline --------------------------------------------------------------
 static int ensure_pipeline(st_handle* h) {
     if (h->d_recs) return ST_OK;
-    // default budget: 64 GiB, at most 40% of the device memory free now
-    uint64_t budget = (uint64_t)65536 << 20;
+    // default budget: 160 GiB, at most half the device memory free now.
+    // Every chunk boundary costs a launch drain (lanes idle while the last
+    // particles of the chunk finish), which matters when a lane holds only a
+    // few particles: 2 x 262,144 particles over 2 s take 3,038 ms in two
+    // chunks (2,048 records) and 2,704 ms in one (4,096).
+    uint64_t budget = (uint64_t)160 << 30;
     if (h->ext.seg_buffer_mb > 0) {
         budget = (uint64_t)h->ext.seg_buffer_mb << 20;
     } else {
         size_t free_b = 0, total_b = 0;
         if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 0)
-            budget = std::min<uint64_t>(budget, (uint64_t)(0.4 * (double)free_b));
+            budget = std::min<uint64_t>(budget, (uint64_t)(0.5 * (double)free_b));
     }
     uint64_t C = budget / ((uint64_t)h->n * sizeof(SegRec));
     // a budget-derived depth stops at 4096 records (seconds of motion per
@@ -512,6 +519,13 @@ static int save_ckpt(st_handle* h, const uint32_t* list, uint32_t n_list) {
 // (k_kin over all n lanes, k_spin over its persistent grid, k_fixup over 64
 // groups); each reads this chunk's list length and failure count on the
 // device.  The graph holds the run's arguments, so it is captured per call.
+// The kinematics kernel over n_lanes lanes, kbs lanes per workgroup.
+static void launch_kin(st_handle* h, const DevState& D, const PipeArgs& A, uint32_t n_lanes, int kbs) {
+    const dim3 grid((n_lanes + kbs - 1) / kbs);
+    if (h->batch) hipLaunchKernelGGL(k_kin<true>, grid, dim3(kbs), 0, h->stream, h->P, D, A);
+    else hipLaunchKernelGGL(k_kin<false>, grid, dim3(kbs), 0, h->stream, h->P, D, A);
+}
+
 static int run_pipeline_graph(st_handle* h, PipeArgs A, SpinFn spin, int kbs) {
     const int K = std::min(64, (h->ext.graph_chunks + 1) / 2 * 2);
     hipLaunchKernelGGL(k_graph_start, dim3(1), dim3(1), 0, h->stream, h->d_small, h->n);
@@ -525,10 +539,7 @@ static int run_pipeline_graph(st_handle* h, PipeArgs A, SpinFn spin, int kbs) {
         A.list = h->d_list[c & 1];
         A.next_list = h->d_list[(c & 1) ^ 1];
         hipLaunchKernelGGL(k_chunk_head, dim3(1), dim3(1), 0, h->stream, h->d_small);
-        if (h->batch)
-            hipLaunchKernelGGL(k_kin<true>, dim3((h->n + kbs - 1) / kbs), dim3(kbs), 0, h->stream, h->P, h->D, A);
-        else
-            hipLaunchKernelGGL(k_kin<false>, dim3((h->n + kbs - 1) / kbs), dim3(kbs), 0, h->stream, h->P, h->D, A);
+        launch_kin(h, h->D, A, h->n, kbs);
         e = hipGetLastError();
         if (e == hipSuccess && spin) e = spin(h, dim3(h->grid_spin), A);
         if (e == hipSuccess) {
@@ -610,10 +621,7 @@ static int run_pipeline(st_handle* h, RunArgs R) {
         // [0] spin queue, [1] next count, [2] failures, [4 + b] queue of spectral block b
         HIPCHK(h, hipMemsetAsync(h->d_small, 0, 8 * sizeof(uint32_t), h->stream));
         if (h->ktime_on) HIPCHK(h, hipEventRecord(h->kev[0], h->stream));
-        if (h->batch)
-            hipLaunchKernelGGL(k_kin<true>, dim3((n_list + kbs - 1) / kbs), dim3(kbs), 0, h->stream, h->P, h->D, A);
-        else
-            hipLaunchKernelGGL(k_kin<false>, dim3((n_list + kbs - 1) / kbs), dim3(kbs), 0, h->stream, h->P, h->D, A);
+        launch_kin(h, h->D, A, n_list, kbs);
         HIPCHK(h, hipGetLastError());
         if (h->ktime_on) HIPCHK(h, hipEventRecord(h->kev[1], h->stream));
         if (h->ckpt_on && (r = save_ckpt(h, A.list, n_list))) return r;
@@ -676,15 +684,15 @@ static int run_pipeline(st_handle* h, RunArgs R) {
 // ---- fused engine ------------------------------------------------------------
 static int run_fused(st_handle* h, RunArgs R, bool persist) {
     if (persist) {
-        int r = persistent_grid(h, fused_symbol(h->ext.variant, h->opt.integratorType), &h->grid_fused);
+        int r = persistent_grid(h, fused_symbol(h->ext.variant, h->opt.integratorType, h->any_pulse), &h->grid_fused);
         if (r) return r;
         HIPCHK(h, hipMemsetAsync(h->d_small, 0, sizeof(uint32_t), h->stream));
         const int grid = std::min<int>(h->grid_fused, (int)((h->n + 255) / 256));
-        hipError_t e = pick_fused<true>(h->ext.variant, h->opt.integratorType)(h, dim3(grid), dim3(256), R);
+        hipError_t e = pick_fused<true>(h->ext.variant, h->opt.integratorType, h->any_pulse)(h, dim3(grid), dim3(256), R);
         if (e != hipSuccess) return fail(h, ST_E_HIP, "engine launch failed: %s", hipGetErrorString(e));
     } else {
         const int bs = h->ext.block_size > 0 ? std::min(256, (h->ext.block_size + 63) / 64 * 64) : 256;
-        hipError_t e = pick_fused<false>(h->ext.variant, h->opt.integratorType)(
+        hipError_t e = pick_fused<false>(h->ext.variant, h->opt.integratorType, h->any_pulse)(
             h, dim3((h->n + bs - 1) / bs), dim3(bs), R);
         if (e != hipSuccess) return fail(h, ST_E_HIP, "engine launch failed: %s", hipGetErrorString(e));
     }
@@ -1275,10 +1283,7 @@ ST_API int st_adjoint(st_handle* h, const void* blocks_dev, const double* seeds,
         A.list = c.list;
         A.n_list = c.n_list;
         const dim3 grid((c.n_list + bs - 1) / bs);
-        if (h->batch)
-            hipLaunchKernelGGL(k_kin<true>, grid, dim3(bs), 0, h->stream, h->P, DR, A);
-        else
-            hipLaunchKernelGGL(k_kin<false>, grid, dim3(bs), 0, h->stream, h->P, DR, A);
+        launch_kin(h, DR, A, c.n_list, bs);
         HIPCHK(h, hipGetLastError());
         J.list = c.list;
         J.n_list = c.n_list;

@@ -138,7 +138,9 @@ __device__ __forceinline__ void poison_and_count(Lane& L) {
 
 enum { PH_DONE = 0, PH_OPEN = 1, PH_EVENT = 2, PH_INT = 3, PH_CLOSE = 4 };
 
-template <int VAR, int INTEG, bool PERSIST>
+// PULSE = false: the fast DOP853 trial's pulse-free form (handles without a
+// pulse term), the same form the pipeline's spin kernel takes for them
+template <int VAR, int INTEG, bool PERSIST, bool PULSE = true>
 __global__ __launch_bounds__(256) void k_engine(const SimParams P, const DevState D, const RunArgs R) {
     const Phys& F = P.ph;
     Lane L;
@@ -222,7 +224,7 @@ __global__ __launch_bounds__(256) void k_engine(const SimParams P, const DevStat
             int st;
             if (INTEG == INT_DOP853) {
                 if (VAR == VAR_EXACT) st = dop_trial_exact(P, F, dop, L.S, L.tempH, seg, L.acc, L.rej);
-                else st = dop_trial_fast(P, F, dop, L.S, L.tempH, seg, poly, L.acc, L.rej);
+                else st = dop_trial_fast<PULSE>(P, F, dop, L.S, L.tempH, seg, poly, L.acc, L.rej);
             } else if (INTEG == INT_RK45) {
                 st = rk_trial<VAR>(P, F, rk, L.S, L.tempH, seg, poly, L.acc, L.rej);
             } else {

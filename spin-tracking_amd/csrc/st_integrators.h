@@ -292,30 +292,50 @@ ST_HD inline int dop_trial_fast_t(const SimParams& P, const Phys& F, DopState& D
     D.nstep++;
     const double x = D.x;
     const double tx = x - S.t0;
-    const Poly hq{d3{h * q.w0.x, h * q.w0.y, h * q.w0.z}, d3{h * q.w1.x, h * q.w1.y, h * q.w1.z},
+    // Stages yield K = h*k.  Without a pulse, the field polynomial is scaled
+    // by h once per trial (hq); with one, the trial scales each k instead (3
+    // multiplies per stage), which keeps the 18 registers of the scaled copy
+    // free for the pulse's own state.
+    Poly hq = q;
+    if constexpr (!PULSE)
+        hq = Poly{d3{h * q.w0.x, h * q.w0.y, h * q.w0.z}, d3{h * q.w1.x, h * q.w1.y, h * q.w1.z},
                   d3{h * q.w2.x, h * q.w2.y, h * q.w2.z}, q.pag, q.pw, q.paxis, q.pon};
+    auto hk = [&](const d3& k) -> d3 {
+        if constexpr (PULSE) return d3{h * k.x, h * k.y, h * k.z};
+        else return k;
+    };
     const d3 K1{h * D.k1.x, h * D.k1.y, h * D.k1.z};
     // The pulse a*cos(w*t) at the stage times t = x + c*h (adjoint: at
     // t_end - (x + c*h), with the field negated): one sincos per trial at the
-    // step start, then cos(phi0 +- w*c*h) by angle addition with the small
-    // stage offset's Taylor sincos; a direct cos when w*h is not small.
+    // step start, then cos(phi0 +- w*c*h) by angle addition with the stage
+    // offset's Taylor sincos.  An offset above 0.5 rad (a step too long for
+    // the pulse, which the controller rejects) is halved until it is below,
+    // and its sine and cosine doubled back (a few ulp per doubling) -- a
+    // smaller register footprint than a general cos in every stage.
     double pC = 1.0, pS = 0.0, hpag = 0.0;
-    bool psmall = true;
     const bool pon = PULSE && q.pon;
     if (pon) {
-        hpag = (ADJ ? -h : h) * q.pag;
+        hpag = ADJ ? -q.pag : q.pag;
         sincos(q.pw * (ADJ ? (Q->t_end - x) : x), &pS, &pC);
-        psmall = fabs(q.pw * h) <= 0.5;
     }
     auto pcos = [&](double c) -> double {
-        if (psmall) {
-            double sn, cs;
-            small_sincos(q.pw * (c * h), sn, cs);
-            return ADJ ? pC * cs + pS * sn : pC * cs - pS * sn;
+        double a = q.pw * (c * h), sn, cs;
+        int halvings = 0;
+        // (bounded: any finite angle is below 0.5 after 1100 halvings; a
+        // non-finite one leaves the loop there and yields NaN)
+        while (fabs(a) > 0.5 && halvings < 1100) {
+            a *= 0.5;
+            ++halvings;
         }
-        return cos(q.pw * (ADJ ? Q->t_end - (x + c * h) : x + c * h));
+        small_sincos(a, sn, cs);
+        for (; halvings > 0; --halvings) {
+            const double s2 = 2.0 * sn * cs;
+            cs = 1.0 - 2.0 * sn * sn;
+            sn = s2;
+        }
+        return ADJ ? pC * cs + pS * sn : pC * cs - pS * sn;
     };
-#define KR(C, YY) poly_rhs<PULSE>(tx + (C) * h, (YY), hq, pon ? hpag * pcos(C) : 0.0)
+#define KR(C, YY) hk(poly_rhs<PULSE>(tx + (C) * h, (YY), hq, pon ? hpag * pcos(C) : 0.0))
     // adjoint quadrature of this trial: sum_i b_i Y_i s_i^n
     d3 q0{0.0, 0.0, 0.0}, q1{0.0, 0.0, 0.0}, q2{0.0, 0.0, 0.0};
     auto quad = [&](double b, double si, const d3& Y) {
@@ -376,7 +396,7 @@ ST_HD inline int dop_trial_fast_t(const SimParams& P, const Phys& F, DopState& D
           y.z + (a121 * K1.z + a124 * K4.z + a125 * K5.z + a126 * K6.z + a127 * K7.z + a128 * K8.z + a129 * K9.z + a1210 * K10.z + a1211 * K11.z));
     quad(b12, tx + h, yy);
     const double c12 = pon ? pcos(1.0) : 0.0;
-    K12 = poly_rhs<PULSE>(tx + h, yy, hq, hpag * c12);
+    K12 = hk(poly_rhs<PULSE>(tx + h, yy, hq, hpag * c12));
 #undef KR
     d3 bs;
     ST_V3(bs, b1 * K1.x + b6 * K6.x + b7 * K7.x + b8 * K8.x + b9 * K9.x + b10 * K10.x + b11 * K11.x + b12 * K12.x,

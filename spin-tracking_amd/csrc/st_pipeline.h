@@ -167,9 +167,26 @@ __device__ __forceinline__ void put_rec(char* recs, uint32_t C, uint32_t pid, ui
 // One step of the kinematic state machine: opens an interval if needed,
 // runs one event of the reference's loop body, and returns the record that
 // describes it (or an R_EMPTY interval close).  Shared by k_kin and k_fixup.
-__device__ __forceinline__ uint64_t kin_step(const SimParams& P, const Phys& F, const RunArgs& R, int32_t integ, KinLane& L,
-                                             double& rt, d3& rp, d3& rv) {
-    bool first = false;
+// It runs in three parts:
+//   kin_begin  opens the interval, the collision time and the move, and names
+//              the re-emission the event needs, if any;
+//   run_job    that re-emission (diffuse_job / gas_job, st_device.h);
+//   kin_end    sanity check, watchdog, record flags, interval close.
+struct KinEvent {
+    bool first;      // the event opened an interval
+    bool empty;      // no event: the interval closes at once (rt/rp/rv, flags set)
+    bool gas_draw;   // the gas job draws the next gas time first
+    int job;         // JOB_NONE / JOB_GAS / JOB_DIFFUSE
+    double Vel;      // post-flight speed (the re-emissions' input)
+};
+
+__device__ __forceinline__ void kin_begin(const SimParams& P, const Phys& F, const RunArgs& R, KinLane& L, KinEvent& E,
+                                          double& rt, d3& rp, d3& rv) {
+    E.first = false;
+    E.empty = false;
+    E.gas_draw = false;
+    E.job = JOB_NONE;
+    E.Vel = 0.0;
     if (!L.open) {
         // src/particle.cpp:748-752: tf = nextTOut; finished = false; tempH = h
         const double T = target_of(R, L.k);
@@ -190,23 +207,30 @@ __device__ __forceinline__ uint64_t kin_step(const SimParams& P, const Phys& F, 
             kin_block(R, L);
             L.k++;
             if (L.k >= R.n_targets) L.done = true;
-            return R_EMPTY | R_IEND;
+            E.empty = true;
+            return;
         }
         L.tf = tgt;
         L.K.finished = false;
         L.n_events = 0;
         L.open = true;
-        first = true;
+        E.first = true;
     }
-    // src/particle.cpp:755-758
-    next_collision(P, F, L.tf, L.K, L.rng);
-    move_and_collide(P, F, L.K, L.rng);
+    // src/particle.cpp:755-756, the re-emission left to the job
+    next_collision_core(P, F, L.tf, L.K, L.rng, E.gas_draw);
+    E.job = move_core(P, F, L.K, L.rng, E.Vel);
+}
+
+__device__ __forceinline__ uint64_t kin_end(const SimParams& P, const RunArgs& R, int32_t integ, KinLane& L,
+                                            const KinEvent& E, double& rt, d3& rp, d3& rv) {
+    if (E.empty) return R_EMPTY | R_IEND;
+    // src/particle.cpp:757
     sanity(P, L.K);
     if (++L.n_events > R.max_events && L.K.failure == 0) {
         L.K.stop = true;
         L.K.failure = FAIL_WATCHDOG;
     }
-    uint64_t fl = (first ? R_FIRST : 0) | ((uint64_t)(uint8_t)L.K.coll_type << 8) |
+    uint64_t fl = (E.first ? R_FIRST : 0) | ((uint64_t)(uint8_t)L.K.coll_type << 8) |
                   ((uint64_t)(uint8_t)L.K.wall_hit << 16);
     if (L.K.failure == 0 && integ != INT_NONE) fl |= R_INTEGRATE;
     rt = L.K.t;
@@ -228,6 +252,19 @@ __device__ __forceinline__ uint64_t kin_step(const SimParams& P, const Phys& F, 
         if (L.k >= R.n_targets) L.done = true;
     }
     return fl;
+}
+
+__device__ __forceinline__ void run_job(const Phys& F, KinLane& L, const KinEvent& E) {
+    if (E.job == JOB_DIFFUSE) diffuse_job(L.K.v, L.K.stop, L.K.wall_hit, E.Vel, L.rng);
+    else if (E.job == JOB_GAS) gas_job(L.K.v, L.K.next_gas, E.gas_draw, E.Vel, F.tc, F.sqrtKT_m, F.dist, L.rng);
+}
+
+__device__ __forceinline__ uint64_t kin_step(const SimParams& P, const Phys& F, const RunArgs& R, int32_t integ, KinLane& L,
+                                             double& rt, d3& rp, d3& rv) {
+    KinEvent E;
+    kin_begin(P, F, R, L, E, rt, rp, rv);
+    if (!E.empty) run_job(F, L, E);
+    return kin_end(P, R, integ, L, E, rt, rp, rv);
 }
 
 // Resets both cursors at the start of an advance / run call.

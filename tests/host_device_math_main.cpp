@@ -42,7 +42,7 @@ static bool close3(const st::d3& a, const or_coords& b) {
     const double s = std::sqrt(b.x * b.x + b.y * b.y + b.z * b.z);
     return std::fabs(a.x - b.x) <= 1e-15 * s && std::fabs(a.y - b.y) <= 1e-15 * s && std::fabs(a.z - b.z) <= 1e-15 * s;
 }
-static long g_diffuse = 0, g_diffuse_inexact = 0;
+static long g_diffuse = 0, g_diffuse_inexact = 0, g_gas_jobs = 0;
 static bool same_rng(const st::Rng& a, const or_rng& b) {
     return a.x == b.x && a.y == b.y && a.z == b.z && a.w == b.w && same(a.spare, b.spare) && a.has == b.hasSpare;
 }
@@ -152,9 +152,28 @@ int main(int argc, char** argv) {
                 K.v = st::d3{v.x, v.y, v.z};
                 K.coll_type = 'N';
                 K.wall_hit = 'x';
+                // the split order of the kinematics kernel's event (phase A,
+                // then the re-emission job with the deferred gas-time draw,
+                // then phase B) on a copy, which must end identical
+                st::Kin K2 = K;
+                st::Rng drng2 = drng;
                 st::next_collision(P, F, tf, K, drng);
                 st::move_and_collide(P, F, K, drng);
                 st::sanity(P, K);
+                {
+                    bool gas_draw = false;
+                    double Vel;
+                    st::next_collision_core(P, F, tf, K2, drng2, gas_draw);
+                    const int job = st::move_core(P, F, K2, drng2, Vel);
+                    if (job == st::JOB_DIFFUSE) st::diffuse_job(K2.v, K2.stop, K2.wall_hit, Vel, drng2);
+                    else if (job == st::JOB_GAS)
+                        st::gas_job(K2.v, K2.next_gas, gas_draw, Vel, F.tc, F.sqrtKT_m, F.dist, drng2);
+                    else if (gas_draw)
+                        K2.next_gas += st::expo_draw(drng2, F.tc);
+                    st::sanity(P, K2);
+                    check(memcmp(&K, &K2, sizeof K) == 0 && memcmp(&drng, &drng2, sizeof drng) == 0, "split event order", i);
+                    if (job == st::JOB_GAS) ++g_gas_jobs;
+                }
 
                 check(same(K.dt, o_dt) && same(K.t, o_t) && same(K.t_old, o_told) && same(K.next_gas, o_ng),
                       "times", i);
@@ -177,6 +196,7 @@ int main(int argc, char** argv) {
                 ++cases;
             }
     printf("diffuse events %ld (last-bit differences %ld)\n", g_diffuse, g_diffuse_inexact);
+    printf("gas jobs %ld\n", g_gas_jobs);
     printf("cases %ld mismatches %ld\n", cases, g_bad);
     return g_bad ? 1 : 0;
 }

@@ -6,7 +6,10 @@ spare), and one full event -- next collision time (src/particle.cpp:121-257),
 move and collision (:263-388), sanity check (:412-444) -- with gravity on
 and off, specular and diffuse walls, Maxwell-Boltzmann and isotropic gas
 collisions, the interval cap, +-0 / +-inf / NaN / tiny velocities, positions
-on the walls and an apex at the ceiling.  Diffuse re-emission velocities are
+on the walls and an apex at the ceiling; and the same event in the split
+order the kinematics kernel runs it (phase A, the re-emission job with the
+deferred gas-time draw, phase B), which must end identical.  Diffuse
+re-emission velocities are
 compared to 1e-15 relative (sin/cos vs sincos in the last bit); the RNG
 stream after them stays exact.  No GPU is used.
 """
@@ -34,3 +37,6 @@ def test_device_kinematics_and_rng_match_oracle(oracle, tmp_path):
     # the diffuse branch was exercised
     n_diffuse = int(r.stdout.split("diffuse events ")[1].split()[0])
     assert n_diffuse > 100000
+    # the split order (phase A, re-emission job, phase B) of the kinematics
+    # kernel ran on every event, gas jobs included
+    assert int(r.stdout.split("gas jobs ")[1].split()[0]) > 100000
