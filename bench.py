@@ -556,14 +556,24 @@ def bench_adjoint(args, variant):
                 r[st._lib.GRAD_PARAMS[i]] = abs(grad[i] - fd) / abs(fd)
             rel[step] = r
         best = {k: min(r[k] for r in rel.values()) for k in rel[1e-3]}
-        # the accuracy check: the same physics and loss with fixed steps,
-        # where the forward map is smooth in theta
-        fixed = adjoint_fd_fixed_step(variant)
-        out["fd_check"] = {"case": fixed["workload"],
-                           "params_within_1e-6_best_step": fixed["params_within_1e-6_best_step"],
-                           "params_within_1e-6_or_loss_resolution": fixed["params_within_1e-6_or_resolution"],
-                           "params": fixed["params"]}
-        out["fd_fixed_step"] = fixed
+        # the accuracy check: the differentiation workload's own ensemble
+        # (131,072 neutrons, 50 s) and loss with fixed steps (h = hmin), where
+        # the forward map is smooth in theta, differenced at the stated 1e-3
+        # of each parameter; and a smaller case with a step ladder
+        # (plus 1e-2 and 1e-4: B0.x's difference carries the loss's curvature
+        # at 1e-3, and the weakest couplings change the loss by a few ulp
+        # there, so each parameter is also reported at its best step)
+        c5 = adjoint_fd_fixed_step(variant, n=args.adjoint_particles, tf=50.0, h=args.adjoint_fd_h,
+                                   steps=(1e-3, 1e-2, 1e-4))
+        rel3 = c5["rel_err_step_1e-3"]
+        out["fd_check"] = {"case": c5["workload"],
+                           "params_within_1e-6_step_1e-3": sum(v < 1e-6 for v in rel3.values()),
+                           "params_within_1e-6": c5["params_within_1e-6_best_step"],
+                           "params_within_1e-6_or_loss_resolution": c5["params_within_1e-6_or_resolution"],
+                           "params": c5["params"], "rel_err_step_1e-3": rel3,
+                           "rel_err_best_step": c5["rel_err_best_step"],
+                           "loss_resolution_step_1e-3": c5["loss_resolution_step_1e-3"]}
+        out["fd_fixed_step"] = adjoint_fd_fixed_step(variant)
         # the adaptive 50 s run: its loss carries a ~1e-12 noise floor from
         # accept/reject decisions that move with theta, so weakly coupled
         # parameters resolve only to that floor (informational)
@@ -579,7 +589,7 @@ def bench_adjoint(args, variant):
     return out
 
 
-def adjoint_fd_fixed_step(variant, n=4096, tf=10.0, h=1e-4):
+def adjoint_fd_fixed_step(variant, n=4096, tf=10.0, h=1e-4, steps=(1e-3, 1e-4, 1e-5)):
     """The same physics and loss with fixed integration steps (hmin = h, so
     the steps stay at h), which make the forward map smooth in theta: central
     differences at 1e-3, 1e-4 and 1e-5 of each parameter against the adjoint
@@ -613,7 +623,7 @@ def adjoint_fd_fixed_step(variant, n=4096, tf=10.0, h=1e-4):
     for i in ADJ_PARAMS:
         name = st._lib.GRAD_PARAMS[i]
         errs = []
-        for step in (1e-3, 1e-4, 1e-5):
+        for step in steps:
             d = step * abs(theta[i])
             tp, tm = theta.copy(), theta.copy()
             tp[i] += d
@@ -624,7 +634,8 @@ def adjoint_fd_fixed_step(variant, n=4096, tf=10.0, h=1e-4):
                 res[name] = 4 * float(np.spacing(abs(Lval))) / (2 * d * abs(fd))
         rel3[name] = errs[0]
         best[name] = min(errs)
-    return {"workload": f"{n} neutrons, {tf:g} s, fixed steps of {h:g} s, same loss", "loss": Lval,
+    return {"workload": f"{n} neutrons, {tf:g} s, fixed steps of {h:g} s, same loss, differences at "
+                        f"{', '.join(f'{x:g}' for x in steps)} of each parameter", "loss": Lval,
             "rel_err_step_1e-3": rel3, "rel_err_best_step": best, "loss_resolution_step_1e-3": res,
             "params_within_1e-6_best_step": sum(v < 1e-6 for v in best.values()),
             "params_within_1e-6_or_resolution": sum(best[k] < max(1e-6, res[k]) for k in best),
@@ -718,6 +729,8 @@ def main():
     ap.add_argument("--dressing-per", type=int, default=262144, help="particles per species")
     ap.add_argument("--no-adjoint-fd", action="store_true", help="skip its finite-difference check")
     ap.add_argument("--adjoint-particles", type=int, default=131072)
+    ap.add_argument("--adjoint-fd-h", type=float, default=5e-4,
+                    help="fixed step of the finite-difference check on the differentiation ensemble")
     ap.add_argument("--adjoint-fd-ladder", action="store_true",
                     help="also difference at 1e-5 of each parameter")
     ap.add_argument("--sweep-configs", type=int, default=64)
@@ -869,8 +882,10 @@ def main():
         "dressing_batched_over_plain": dressing["batched_over_plain"] if dressing else None,
         "dressing_batched_spin_frac": dressing["batched_spin_frac_fp64_peak"] if dressing else None,
         "adjoint_over_forward": adj["adjoint_over_forward"] if adj else None,
-        "adjoint_fd_within_1e-6": (adj["fd_check"]["params_within_1e-6_or_loss_resolution"]
-                                   if adj and adj.get("fd_check") else None),
+        "adjoint_fd_within_1e-6": (adj["fd_check"]["params_within_1e-6"] if adj and adj.get("fd_check") else None),
+        "adjoint_fd_within_1e-6_or_resolution": (adj["fd_check"]["params_within_1e-6_or_loss_resolution"]
+                                                 if adj and adj.get("fd_check") else None),
+        "adjoint_fd_case": adj["fd_check"]["case"] if adj and adj.get("fd_check") else None,
         "cpu_all_threads": cpu["value"] if cpu else None,
         "cpu_one_thread": cpu["single_thread"]["value"] if cpu else None,
         "gpu_over_cpu_all_threads": value / cpu["value"] if cpu else None,

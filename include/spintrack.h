@@ -104,6 +104,9 @@ typedef struct st_ext {
     uint32_t max_events;    /* events one particle may take in one output interval before it is
                                stopped with failureState ST_FAIL_WATCHDOG; 0 = ST_DEFAULT_MAX_EVENTS,
                                0xffffffff = no bound (the reference's loop) */
+    uint32_t adjoint_window; /* st_adjoint: checkpoint chunks replayed and walked back per adjoint
+                                launch (the same gradient for any value); 0 = as many as a record
+                                window of at most 64 GiB and a quarter of the free memory holds */
 } st_ext;
 
 /* include/particle.h:52-59 */

@@ -71,7 +71,8 @@ __global__ __launch_bounds__(256) void k_adj_begin(const DevState A, uint32_t n,
     for (int f = 0; f < N_GRAD; ++f) A.st(A_G0 + f, o, 0.0);
 }
 
-template <bool BATCH>
+// PULSE = false: the fast trial's pulse-free form (runs without a pulse term)
+template <bool BATCH, bool PULSE>
 __global__ __launch_bounds__(256) void k_adj(const SimParams P, const DevState D, const AdjArgs A) {
 #pragma clang fp contract(fast)
     const uint32_t li = blockIdx.x * blockDim.x + threadIdx.x;
@@ -137,7 +138,7 @@ __global__ __launch_bounds__(256) void k_adj(const SimParams P, const DevState D
         dop.nstep = 0;
         dop.k1 = poly_rhs(0.0, u, qb, qb.pon ? -(qb.pag * cos(qb.pw * rec.t)) : 0.0);
         int st = 0;
-        while ((st = dop_trial_fast_t<true>(P, F, dop, u, h, sb, qb, nacc, nrej, &Q)) == 0) {
+        while ((st = dop_trial_fast_t<true, PULSE>(P, F, dop, u, h, sb, qb, nacc, nrej, &Q)) == 0) {
         }
         if (st < 0) {
             fail = st;

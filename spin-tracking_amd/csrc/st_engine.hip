@@ -222,6 +222,9 @@ struct st_handle {
     uint32_t* d_seg_count = nullptr;
     char* d_backup = nullptr;
     uint32_t* d_fail = nullptr;
+    // the adjoint's replay window (several checkpoint chunks of records)
+    char* d_adj_recs = nullptr;
+    size_t adj_recs_bytes = 0;
     // segment trace of one particle
     double* d_trace = nullptr;
     uint32_t* d_trace_count = nullptr;
@@ -1006,7 +1009,7 @@ ST_API void st_destroy(st_handle* h) {
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     void* bufs[] = {h->state, h->d_targets, h->d_stop_at, h->d_work, h->d_small, h->d_trace, h->d_trace_count,
                     h->d_recs, h->d_list[0], h->d_list[1], h->d_seg_count, h->d_backup, h->d_fail,
-                    h->d_phys, h->d_cfg_off, h->d_cfg_seed, h->d_adj, h->d_replay, h->d_scratch,
+                    h->d_phys, h->d_cfg_off, h->d_cfg_seed, h->d_adj, h->d_replay, h->d_scratch, h->d_adj_recs,
                     h->spec.agg, h->spec.nsamp, h->d_spec, h->d_spec_n, h->d_spec_part};
     free_ckpts(h);
     for (void* b : bufs)
@@ -1272,27 +1275,72 @@ ST_API int st_adjoint(st_handle* h, const void* blocks_dev, const double* seeds,
     J.seeds = d_seeds;
     J.n_targets = K;
     J.phys = h->d_phys;
-    for (size_t m = h->ckpts.size(); m-- > 0;) {
-        const st_handle::Ckpt& c = h->ckpts[m];
-        // replay chunk m's kinematics from its start state: the records are
-        // bit-identical to the forward run's
+    // Checkpoint groups: the replay runs G consecutive checkpoint chunks in one
+    // stretch, from the first one's start state, into a window of G*C records
+    // per particle (the same records the forward run wrote chunk by chunk:
+    // its kinematics continue across chunk boundaries unchanged), and one
+    // adjoint launch walks the window backwards.  Fewer, longer launches
+    // spend less time draining (one particle per lane at this ensemble
+    // size, so a launch lasts as long as its slowest particle).  G is what a
+    // record window of at most 64 GiB, and a quarter of the free device
+    // memory, holds.
+    const size_t n_ck = h->ckpts.size();
+    uint32_t G = 1;
+    {
+        size_t free_b = 0, total_b = 0;
+        uint64_t budget = (uint64_t)64 << 30;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 0)
+            budget = std::min<uint64_t>(budget, (uint64_t)(0.25 * (double)free_b) + h->adj_recs_bytes);
+        const uint64_t per_chunk = (uint64_t)h->n * h->rec_cap * sizeof(SegRec);
+        G = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(n_ck, budget / per_chunk));
+        if (h->ext.adjoint_window > 0) G = (uint32_t)std::min<uint64_t>(n_ck, h->ext.adjoint_window);
+    }
+    char* recs = h->d_recs;
+    uint32_t* seg_count = h->d_seg_count;
+    if (G > 1) {
+        const size_t need = (size_t)h->n * G * h->rec_cap * sizeof(SegRec);
+        if (h->adj_recs_bytes < need) {
+            if (h->d_adj_recs) HIPCHK(h, hipFree(h->d_adj_recs));
+            h->d_adj_recs = nullptr;
+            h->adj_recs_bytes = 0;
+            HIPCHK(h, hipMalloc(&h->d_adj_recs, need));
+            h->adj_recs_bytes = need;
+        }
+        recs = h->d_adj_recs;
+    }
+    A.recs = recs;
+    A.seg_count = seg_count;
+    J.recs = recs;
+    J.seg_count = seg_count;
+    for (size_t hi = n_ck; hi > 0;) {
+        const size_t lo = hi >= G ? hi - G : 0;
+        const st_handle::Ckpt& c = h->ckpts[lo];
+        // replay chunks lo .. hi-1 from chunk lo's start state: the records
+        // are bit-identical to the forward run's
         for (int f : KIN_FIELDS)
             HIPCHK(h, hipMemcpyAsync(DR.f<char>(f), c.state + (size_t)f * h->D.stride, h->D.stride,
                                      hipMemcpyDeviceToDevice, h->stream));
         HIPCHK(h, hipMemsetAsync(small, 0, 4 * sizeof(uint32_t), h->stream));
         A.list = c.list;
         A.n_list = c.n_list;
+        A.C = (uint32_t)(hi - lo) * h->rec_cap;
         const dim3 grid((c.n_list + bs - 1) / bs);
         launch_kin(h, DR, A, c.n_list, bs);
         HIPCHK(h, hipGetLastError());
         J.list = c.list;
         J.n_list = c.n_list;
+        J.C = A.C;   // the window's records per particle: bound and layout stride of both kernels
         J.ck = DevState{c.state, h->D.stride};
-        if (h->batch)
-            hipLaunchKernelGGL(k_adj<true>, grid, dim3(bs), 0, h->stream, h->P, h->D, J);
+        if (h->batch && h->any_pulse)
+            hipLaunchKernelGGL((k_adj<true, true>), grid, dim3(bs), 0, h->stream, h->P, h->D, J);
+        else if (h->batch)
+            hipLaunchKernelGGL((k_adj<true, false>), grid, dim3(bs), 0, h->stream, h->P, h->D, J);
+        else if (h->any_pulse)
+            hipLaunchKernelGGL((k_adj<false, true>), grid, dim3(bs), 0, h->stream, h->P, h->D, J);
         else
-            hipLaunchKernelGGL(k_adj<false>, grid, dim3(bs), 0, h->stream, h->P, h->D, J);
+            hipLaunchKernelGGL((k_adj<false, false>), grid, dim3(bs), 0, h->stream, h->P, h->D, J);
         HIPCHK(h, hipGetLastError());
+        hi = lo;
     }
     // deterministic reduction: per-block partials per configuration, summed
     // on the host in block order

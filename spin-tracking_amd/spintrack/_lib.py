@@ -64,7 +64,8 @@ ENGINE_FUSED = 1
 class Ext(C.Structure):
     _fields_ = [("pulse_axis", C.c_int32), ("variant", C.c_int32), ("block_size", C.c_int32),
                 ("engine", C.c_int32), ("seg_buffer_mb", C.c_int32), ("chunk_records", C.c_int32),
-                ("graph_chunks", C.c_int32), ("spectrum_sum", C.c_int32), ("max_events", C.c_uint32)]
+                ("graph_chunks", C.c_int32), ("spectrum_sum", C.c_int32), ("max_events", C.c_uint32),
+                ("adjoint_window", C.c_uint32)]
 
 
 class Rng(C.Structure):

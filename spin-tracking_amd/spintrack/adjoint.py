@@ -45,10 +45,11 @@ class ForwardRun:
     Checkpoints are taken every ``chunk_records`` trajectory segments."""
 
     def __init__(self, opt: Options, variant: int = L.VARIANT_FAST, chunk_records: int = 256,
-                 pulse_axis: int = 2):
+                 pulse_axis: int = 2, adjoint_window: int = 0):
         self.opt = Options.from_buffer_copy(bytes(opt))
         self.times = np.asarray(output_times(self.opt), dtype=np.float64)
-        self.p = Particle(self.opt, pulse_axis=pulse_axis, variant=variant, chunk_records=chunk_records)
+        self.p = Particle(self.opt, pulse_axis=pulse_axis, variant=variant, chunk_records=chunk_records,
+                          adjoint_window=adjoint_window)
         self.blocks = None
 
     def run(self) -> np.ndarray:

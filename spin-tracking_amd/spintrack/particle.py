@@ -57,12 +57,13 @@ class Particle:
     def __init__(self, opt: Options, pulse_axis: int = 2, variant: int = L.VARIANT_EXACT,
                  block_size: int = 0, engine: int = L.ENGINE_PIPELINE, seg_buffer_mb: int = 0,
                  chunk_records: int = 0, graph_chunks: int = 0, spectrum_sum: int = 0, max_events: int = 0,
-                 library: str = None):
+                 library: str = None, adjoint_window: int = 0):
         self.lib = L.open_library(library) if library else L.load()
         self.opt = Options.from_buffer_copy(bytes(opt))
         self.n = int(opt.numParticles)
         self.ext = Ext(int(pulse_axis), int(variant), int(block_size), int(engine), int(seg_buffer_mb),
-                       int(chunk_records), int(graph_chunks), int(spectrum_sum), int(max_events))
+                       int(chunk_records), int(graph_chunks), int(spectrum_sum), int(max_events),
+                       int(adjoint_window))
         h = C.c_void_p()
         check(self.lib.st_create(C.byref(self.opt), C.byref(self.ext), C.byref(h)), None, self.lib)
         self.h = h
