@@ -551,15 +551,62 @@ struct Poly {
     int32_t paxis, pon;
 };
 
+// Taylor coefficients in x^2 of cos x (row 0: (-1)^k / (2k)!) and of
+// sin(x)/x (row 1: (-1)^k / (2k+1)!), k = 0..10.  The device reads them from
+// constant memory at uniform offsets, so every Horner step is one FMA with a
+// scalar-register operand: written as literals, each coefficient that is
+// not an inline constant costs two extra vector moves per use (a 64-bit
+// literal cannot be an FMA operand, and the accumulating form wants it in
+// vector registers).
+#define ST_TAYLOR_COS 1.0, -1.0 / 2, 1.0 / 24, -1.0 / 720, 1.0 / 40320, -1.0 / 3628800, 1.0 / 479001600, \
+    -1.0 / 87178291200.0, 1.0 / 20922789888000.0, -1.0 / 6402373705728000.0, 1.0 / 2432902008176640000.0
+#define ST_TAYLOR_SINC 1.0, -1.0 / 6, 1.0 / 120, -1.0 / 5040, 1.0 / 362880, -1.0 / 39916800, 1.0 / 6227020800.0, \
+    -1.0 / 1307674368000.0, 1.0 / 355687428096000.0, -1.0 / 121645100408832000.0, 1.0 / 51090942171709440000.0
+#if defined(__HIP_DEVICE_COMPILE__)   // device-only: no host shadow to register
+__constant__ double TAYLOR_DEV[2][11] = {{ST_TAYLOR_COS}, {ST_TAYLOR_SINC}};
+#endif
+constexpr double TAYLOR_HOST[2][11] = {{ST_TAYLOR_COS}, {ST_TAYLOR_SINC}};
+#undef ST_TAYLOR_COS
+#undef ST_TAYLOR_SINC
+
+// sum_{k<N} TAYLOR[row][k] x2^k by Horner, one FMA per term
+template <int ROW, int N>
+ST_HD __forceinline__ double taylor_sum(double x2) {
+#pragma clang fp contract(fast)
+#if defined(__HIP_DEVICE_COMPILE__)
+    // The table through an opaque constant-space copy of its (uniform)
+    // address: the scalar loads stay at the use (hoisted out of the lane
+    // loop, the 22 coefficients would stay live in scalar registers and
+    // spill), and each step is written as the three-operand FMA with the
+    // coefficient in scalar registers (left to itself the compiler picks
+    // the accumulating form and first copies the coefficient into vector
+    // registers, two moves per step).
+    typedef const __attribute__((address_space(4))) double cdouble;
+    cdouble* c = (cdouble*)TAYLOR_DEV[ROW];
+    asm volatile("" : "+s"(c));
+    double a = c[N - 1];
+#pragma unroll
+    for (int k = N - 2; k >= 0; --k) {
+        double r;
+        asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(x2), "v"(a), "s"(c[k]));
+        a = r;
+    }
+    return a;
+#else
+    const double* c = TAYLOR_HOST[ROW];
+    double a = c[N - 1];
+    for (int k = N - 2; k >= 0; --k) a = x2 * a + c[k];
+    return a;
+#endif
+}
+
 // sin and cos of a small angle, |th| <= 0.5, by their Taylor series (terms
 // to th^15 and th^14; the first omitted terms are below 3e-17 relative).
 ST_HD __forceinline__ void small_sincos(double th, double& s, double& c) {
 #pragma clang fp contract(fast)
     const double t2 = th * th;
-    c = 1.0 + t2 * (-1.0 / 2 + t2 * (1.0 / 24 + t2 * (-1.0 / 720 + t2 * (1.0 / 40320 + t2 * (-1.0 / 3628800 +
-        t2 * (1.0 / 479001600 + t2 * (-1.0 / 87178291200.0)))))));
-    s = th * (1.0 + t2 * (-1.0 / 6 + t2 * (1.0 / 120 + t2 * (-1.0 / 5040 + t2 * (1.0 / 362880 + t2 * (-1.0 / 39916800 +
-        t2 * (1.0 / 6227020800.0 + t2 * (-1.0 / 1307674368000.0))))))));
+    c = taylor_sum<0, 8>(t2);
+    s = th * taylor_sum<1, 8>(t2);
 }
 
 // gamma*B along a segment plus the pulse value pv (already scaled like the

@@ -673,34 +673,54 @@ ST_HD __forceinline__ d3 rodriguez(const d3& k, const d3& v) {
 // Fast variant of the two rotations, without square root, division or trig
 // call: cos x and sin(x)/x for x^2 <= 1 from their Taylor series in x^2
 // (terms to x^20 and x^18; the first omitted terms are below 1e-19 of the
-// sum).  A Rodrigues rotation by theta = |k| <= 4 takes them at x = theta/4
-// and doubles twice: for an angle 2x, sin(2x)/(2x) = s c,
-// (1 - cos 2x)/(2x)^2 = s^2/2 (no cancellation) and cos 2x = 1 - (2x)^2 s^2/2
-// (within 6 units in the last place of libm over the range).  A quaternion
-// takes them at its own half angle, up to 1.  Other vectors (larger, zero: the
-// reference's 0/0, or not finite) take the exact form.  The values agree
-// with the exact form to a few units in the last place, well inside the
+// sum), taken at x / 2^d and doubled d times: for an angle 2y,
+// sin(2y)/(2y) = s c, (1 - cos 2y)/(2y)^2 = s^2/2 (no cancellation) and
+// cos 2y = 1 - (2y)^2 s^2/2.  A Rodrigues rotation by theta = |k| takes them
+// at theta/4 and doubles twice (within 6 units in the last place of libm for
+// theta <= 4); a quaternion takes them at its own half angle.  Longer steps
+// (tens of radians in one exponent: a smooth field lets the controller grow
+// the step to the segment's end) halve further and take the extra doublings
+// in a short loop of their own, a few ulp per doubling -- in place of the
+// library's argument reduction, which one such lane would otherwise impose
+// on its whole wave.  Zero vectors (the reference's 0/0), squares above 2^60
+// and non-finite vectors take the exact form.  The values agree with the
+// exact form to a few units in the last place per doubling, well inside the
 // integrator tolerance the fast variant is held to.
 ST_HD __forceinline__ void half_series(double x2, double& c, double& s) {
+    c = taylor_sum<0, 11>(x2);
+    s = taylor_sum<1, 10>(x2);
+}
+constexpr double FAST_ROT_MAX_X2 = 0x1p60;   // at most 31 doublings
+// cos x and sin(x)/x for x^2 = X2 in (0, 2^60]: the series after d >= D0
+// halvings, d the fewest that bring the square to 1 or below
+template <int D0>
+ST_HD __forceinline__ void cos_sinc_doubled(double X2, double& c, double& s) {
 #pragma clang fp contract(fast)
-    c = 1.0 + x2 * (-1.0 / 2 + x2 * (1.0 / 24 + x2 * (-1.0 / 720 + x2 * (1.0 / 40320 + x2 * (-1.0 / 3628800 +
-        x2 * (1.0 / 479001600 + x2 * (-1.0 / 87178291200.0 + x2 * (1.0 / 20922789888000.0 +
-        x2 * (-1.0 / 6402373705728000.0 + x2 * (1.0 / 2432902008176640000.0))))))))));
-    s = 1.0 + x2 * (-1.0 / 6 + x2 * (1.0 / 120 + x2 * (-1.0 / 5040 + x2 * (1.0 / 362880 + x2 * (-1.0 / 39916800 +
-        x2 * (1.0 / 6227020800.0 + x2 * (-1.0 / 1307674368000.0 + x2 * (1.0 / 355687428096000.0 +
-        x2 * (-1.0 / 121645100408832000.0)))))))));
+    int E;
+    (void)frexp(X2, &E);   // X2 < 2^E, so X2 / 4^d < 1 once 2d >= E
+    const int d = E > 2 * D0 ? (E + 1) >> 1 : D0;
+    double x2 = ldexp(X2, -2 * d);
+    half_series(x2, c, s);
+    auto twice = [&]() {
+        const double s2 = s * s;
+        s = s * c;
+        c = 1.0 - (2.0 * x2) * s2;
+        x2 *= 4.0;
+    };
+    for (int i = d; i > D0; --i) twice();
+#pragma unroll
+    for (int i = 0; i < D0; ++i) twice();
 }
 template <int VAR>
 ST_HD __forceinline__ d3 rodriguez_v(const d3& k, const d3& v) {
     if constexpr (VAR == VAR_FAST) {
 #pragma clang fp contract(fast)
         const double t2 = k.x * k.x + k.y * k.y + k.z * k.z;
-        if (t2 > 0.0 && t2 <= 16.0) {
-            // series at theta/4, doubled to theta/2, then to theta
-            double cq, sq;
-            half_series(0.0625 * t2, cq, sq);
-            const double sh = sq * cq;                            // sin(theta/2) / (theta/2)
-            const double ch = 1.0 - (0.25 * t2) * (0.5 * (sq * sq));   // cos(theta/2)
+        if (t2 > 0.0 && t2 <= FAST_ROT_MAX_X2) {
+            // cos(theta/2) and sin(theta/2) / (theta/2) from theta/4 (or
+            // below), then the last doubling to theta
+            double ch, sh;
+            cos_sinc_doubled<1>(0.25 * t2, ch, sh);
             const double sa = sh * ch;           // sin(theta) / theta
             const double f1 = 0.5 * (sh * sh);   // (1 - cos theta) / theta^2
             const double c = 1.0 - t2 * f1;      // cos theta
@@ -719,9 +739,9 @@ ST_HD __forceinline__ Quat rodriguez_quat_v(const d3& k, double dt) {
 #pragma clang fp contract(fast)
         const double hd = 0.5 * dt;
         const double p2 = (k.x * k.x + k.y * k.y + k.z * k.z) * (hd * hd);
-        if (p2 > 0.0 && p2 <= 1.0) {
+        if (p2 > 0.0 && p2 <= FAST_ROT_MAX_X2) {
             double c, sp;
-            half_series(p2, c, sp);
+            cos_sinc_doubled<0>(p2, c, sp);
             const double s = -(hd * sp);
             return Quat{c, k.x * s, k.y * s, k.z * s};
         }
@@ -771,15 +791,53 @@ __constant__ double CFET_ROWS[16][5] = {{ST_G1R}, {ST_G2R}, {ST_G3R}, {ST_G4R}, 
                                         {ST_G5},  {ST_G4},  {ST_G3},  {ST_G2},  {ST_G1}, {ST_H1R},
                                         {ST_H2R}, {ST_H3},  {ST_H2},  {ST_H1}};
 
+// The same rows as moments for the fast variant.  Along a segment gamma*B is
+// the quadratic w(tau) = w0 + w1 tau + w2 tau^2, so an exponent
+// h sum_i c_i w(tau0 + X_i h) equals exactly
+//   M0 [h w(tau0)] + M1 [h^2 w'(tau0)] + M2 [h^3 w2],  Mk = sum_i c_i X_i^k:
+// nine operations per exponent from three vectors formed once per trial,
+// in place of five field samples and fifteen per exponent.  Entries 3..7
+// keep the row itself for the pulse, which is still sampled at the nodes.
+struct CfetMoments {
+    double m[16][8];
+};
+constexpr CfetMoments cfet_moments() {
+    constexpr double rows[16][5] = {{ST_G1R}, {ST_G2R}, {ST_G3R}, {ST_G4R}, {ST_G5R}, {ST_G6},
+                                    {ST_G5},  {ST_G4},  {ST_G3},  {ST_G2},  {ST_G1}, {ST_H1R},
+                                    {ST_H2R}, {ST_H3},  {ST_H2},  {ST_H1}};
+    constexpr double X[5] = {cfet::X1, cfet::X2, cfet::X3, cfet::X4, cfet::X5};
+    CfetMoments M{};
+    for (int r = 0; r < 16; ++r) {
+        for (int i = 0; i < 5; ++i) {
+            M.m[r][0] += rows[r][i];
+            M.m[r][1] += rows[r][i] * X[i];
+            M.m[r][2] += rows[r][i] * X[i] * X[i];
+            M.m[r][3 + i] = rows[r][i];
+        }
+    }
+    return M;
+}
+#if defined(__HIP_DEVICE_COMPILE__)
+__constant__ CfetMoments CFET_MOM = cfet_moments();
+#endif
+constexpr CfetMoments CFET_MOM_HOST = cfet_moments();
+
 // Quaternion integrators, per stage i = 0..5: node A_i, weight W_i of the
 // step, weight E_i of the comparison solution (E_5 unused); [0] RK45
 // (include/coeff.h:178-218: A, CH, C), [1] RKF45 (:220-256: A, C, C).
-__constant__ double QUAT_TAB[2][6][3] = {
-    {{0.0, 47.0 / 450.0, 1.0 / 9.0}, {2.0 / 9.0, 0.0, 0.0}, {1.0 / 3.0, 12.0 / 25.0, 9.0 / 20.0},
-     {3.0 / 4.0, 32.0 / 225.0, 16.0 / 45.0}, {1.0, 1.0 / 30.0, 1.0 / 12.0}, {5.0 / 6.0, 6.0 / 25.0, 0.0}},
-    {{0.0, 16.0 / 135.0, 16.0 / 135.0}, {1.0 / 4.0, 0.0, 0.0}, {3.0 / 8.0, 6656.0 / 12825.0, 6656.0 / 12825.0},
-     {12.0 / 13.0, 28561.0 / 56430.0, 28561.0 / 56430.0}, {1.0, -9.0 / 50.0, -9.0 / 50.0},
-     {0.5, 2.0 / 55.0, 0.0}}};
+#define ST_QUAT_TAB                                                                                            \
+    {{{0.0, 47.0 / 450.0, 1.0 / 9.0}, {2.0 / 9.0, 0.0, 0.0}, {1.0 / 3.0, 12.0 / 25.0, 9.0 / 20.0},               \
+      {3.0 / 4.0, 32.0 / 225.0, 16.0 / 45.0}, {1.0, 1.0 / 30.0, 1.0 / 12.0}, {5.0 / 6.0, 6.0 / 25.0, 0.0}},        \
+     {{0.0, 16.0 / 135.0, 16.0 / 135.0}, {1.0 / 4.0, 0.0, 0.0}, {3.0 / 8.0, 6656.0 / 12825.0, 6656.0 / 12825.0},   \
+      {12.0 / 13.0, 28561.0 / 56430.0, 28561.0 / 56430.0}, {1.0, -9.0 / 50.0, -9.0 / 50.0},                      \
+      {0.5, 2.0 / 55.0, 0.0}}}
+__constant__ double QUAT_TAB[2][6][3] = ST_QUAT_TAB;
+// the same table for compile-time decisions of the fast variant
+struct QuatTab {
+    double q[2][6][3];
+};
+constexpr QuatTab QUAT_TAB_C = {ST_QUAT_TAB};
+#undef ST_QUAT_TAB
 
 // Step-size bounds of the four (e.g. :355-385): the end-of-interval test
 // comes first and skips the hmax/hmin checks, and a fixed step size never
@@ -835,6 +893,54 @@ ST_HD __forceinline__ int control_v2(const SimParams& P, RkState& R, d3& y, cons
     if (!P.fixed) h = h * q_;
     R.prev_ratio = ratio;
     return 0;
+}
+
+// Fast form of the quaternion trials.  A stage whose weight is zero (the
+// second stage of both tableaus) contributes the reference's
+// rodriguez_quat(k, 0): the identity for a finite nonzero k (multiplying by
+// it changes no nonzero component), NaN for a zero or non-finite k (its
+// 0/0); the factor is not formed, only that test is kept (the weights are
+// read from constant memory, so these branches are uniform).  Where
+// W_i = E_i for the first five stages (RKF45) the two chains share their
+// factors: the comparison product is the step's product after stage five.
+template <int INTEG, typename WF>
+ST_HD __forceinline__ int quat_trial_fast(const SimParams& P, RkState& R, d3& y, double& h, double t, WF&& W,
+                                          uint32_t& nacc, uint32_t& nrej) {
+#pragma clang fp contract(fast)
+    constexpr int TI = INTEG == INT_RKF45Q ? 1 : 0;
+    constexpr bool SHARED = QUAT_TAB_C.q[TI][0][1] == QUAT_TAB_C.q[TI][0][2] &&
+                            QUAT_TAB_C.q[TI][1][1] == QUAT_TAB_C.q[TI][1][2] &&
+                            QUAT_TAB_C.q[TI][2][1] == QUAT_TAB_C.q[TI][2][2] &&
+                            QUAT_TAB_C.q[TI][3][1] == QUAT_TAB_C.q[TI][3][2] &&
+                            QUAT_TAB_C.q[TI][4][1] == QUAT_TAB_C.q[TI][4][2];
+    static_assert(QUAT_TAB_C.q[TI][0][1] != 0.0 && QUAT_TAB_C.q[TI][0][2] != 0.0, "first factors start the chains");
+    const double(*Q)[3] = QUAT_TAB[TI];
+    d3 k = W(t);
+    Quat qw = rodriguez_quat_v<VAR_FAST>(k, h * Q[0][1]);
+    Quat qe = SHARED ? qw : rodriguez_quat_v<VAR_FAST>(k, h * Q[0][2]);
+    bool degenerate = false;   // a zero-weight factor met a zero or non-finite k
+#pragma unroll 1
+    for (int i = 1; i < 6; ++i) {
+        k = W(t + Q[i][0] * h);
+        const double ww = Q[i][1], we = i < 5 ? Q[i][2] : 1.0;
+        if (ww == 0.0 || we == 0.0) {
+            const double t2 = k.x * k.x + k.y * k.y + k.z * k.z;
+            degenerate = degenerate || !(t2 > 0.0 && t2 <= DBL_MAX_D);
+        }
+        if (ww != 0.0) qw = q_mult(rodriguez_quat_v<VAR_FAST>(k, h * ww), qw);
+        if (i < 5) {
+            if constexpr (SHARED) qe = qw;
+            else if (we != 0.0) qe = q_mult(rodriguez_quat_v<VAR_FAST>(k, h * we), qe);
+        }
+    }
+    if (degenerate) {
+        const double nan = __builtin_nan("");
+        qw = qe = Quat{nan, nan, nan, nan};
+    }
+    const d3 ws = qv_mult(qw, y);
+    const d3 te = qv_mult(qe, y);
+    const d3 d{ws.x - te.x, ws.y - te.y, ws.z - te.z};
+    return control_v2<VAR_FAST>(P, R, y, ws, h, len3(d), nacc, nrej);
 }
 
 template <int VAR, int INTEG>
@@ -897,6 +1003,7 @@ ST_HD inline int rk2_trial(const SimParams& P, const Phys& Fp, RkState& R, d3& y
         // k_i = gamma*B(t + A_i h); the step rotates y by the product of
         // R(k_i, h W_i), k1 first; the comparison solution drops the sixth
         // factor and uses weights E_i (RK45: W = CH, E = C; RKF45: W = E = C)
+        if constexpr (VAR == VAR_FAST) return quat_trial_fast<INTEG>(P, R, y, h, t, W, nacc, nrej);
         const double(*Q)[3] = QUAT_TAB[INTEG == INT_RKF45Q ? 1 : 0];
         d3 k = W(t);
         Quat qw = rodriguez_quat_v<VAR>(k, h * Q[0][1]);
@@ -914,6 +1021,47 @@ ST_HD inline int rk2_trial(const SimParams& P, const Phys& Fp, RkState& R, d3& y
     } else {
         static_assert(INTEG == INT_MAGNUS, "integrator type");
         using namespace cfet;
+        if constexpr (VAR == VAR_FAST) {
+#pragma clang fp contract(fast)
+            // the exponents from the segment polynomial's moments (CFET_MOM)
+            const double tau = t - S.t0, h2 = h * h, h3 = h2 * h, tt = 2.0 * tau;
+            const d3 u0{h * (q.w0.x + tau * (q.w1.x + tau * q.w2.x)), h * (q.w0.y + tau * (q.w1.y + tau * q.w2.y)),
+                        h * (q.w0.z + tau * (q.w1.z + tau * q.w2.z))};
+            const d3 u1{h2 * (q.w1.x + tt * q.w2.x), h2 * (q.w1.y + tt * q.w2.y), h2 * (q.w1.z + tt * q.w2.z)};
+            const d3 u2{h3 * q.w2.x, h3 * q.w2.y, h3 * q.w2.z};
+            // h * pulse at the five nodes
+            double hp[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+            const bool pon = q.pon;
+            if (pon) {
+                const double X[5] = {X1, X2, X3, X4, X5};
+#pragma unroll
+                for (int i = 0; i < 5; ++i) hp[i] = h * (q.pag * cos(q.pw * (t + X[i] * h)));
+            }
+            auto expo = [&](int r) -> d3 {
+#if defined(__HIP_DEVICE_COMPILE__)
+                const double* m = CFET_MOM.m[r];
+#else
+                const double* m = CFET_MOM_HOST.m[r];
+#endif
+                d3 k{m[0] * u0.x + m[1] * u1.x + m[2] * u2.x, m[0] * u0.y + m[1] * u1.y + m[2] * u2.y,
+                     m[0] * u0.z + m[1] * u1.z + m[2] * u2.z};
+                if (pon) {
+                    const double pe = m[3] * hp[0] + m[4] * hp[1] + m[5] * hp[2] + m[6] * hp[3] + m[7] * hp[4];
+                    k.x += q.paxis == 0 ? pe : 0.0;
+                    k.y += q.paxis == 1 ? pe : 0.0;
+                    k.z += q.paxis == 2 ? pe : 0.0;
+                }
+                return k;
+            };
+            d3 y8 = y, y6 = y;
+#pragma unroll 1
+            for (int r = 0; r < 11; ++r) {
+                y8 = rodriguez_v<VAR>(expo(r), y8);
+                if (r < 5) y6 = rodriguez_v<VAR>(expo(11 + r), y6);
+            }
+            const d3 d{y8.x - y6.x, y8.y - y6.y, y8.z - y6.z};
+            return control_v2<VAR>(P, R, y, y8, h, len3(d), nacc, nrej);
+        }
         const d3 B[5] = {W(t + X1 * h), W(t + X2 * h), W(t + X3 * h), W(t + X4 * h), W(t + X5 * h)};
         // order 8: k11 (row 1 reversed) acts first ... k6 (row 6) ... k1
         // (row 1); order 6: k5 (row 1 reversed), k4 (row 2 reversed), k3,
