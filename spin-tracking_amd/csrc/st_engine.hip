@@ -208,7 +208,7 @@ struct st_handle {
     double* d_stop_at = nullptr;
     bool have_stop_at = false;
     unsigned long long* d_work = nullptr;
-    uint32_t* d_small = nullptr;   // [0] queue, [1] next_count, [2] fail_count
+    uint32_t* d_small = nullptr;   // [0] queue, [1] next_count, [2] fail_count, [3] overflow pages taken
     int device = -1;               // the device current at st_create
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
@@ -218,6 +218,10 @@ struct st_handle {
     // pipeline buffers
     char* d_recs = nullptr;
     uint32_t rec_cap = 0;          // C
+    char* d_ovf = nullptr;         // overflow pages (plain pipeline runs)
+    uint32_t* d_ovf_first = nullptr;
+    uint32_t* d_ovf_link = nullptr;
+    uint32_t ovf_pages = 0;
     uint32_t* d_list[2] = {nullptr, nullptr};
     uint32_t* d_seg_count = nullptr;
     char* d_backup = nullptr;
@@ -476,13 +480,33 @@ static int ensure_pipeline(st_handle* h) {
         if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 0)
             budget = std::min<uint64_t>(budget, (uint64_t)(0.5 * (double)free_b));
     }
-    uint64_t C = budget / ((uint64_t)h->n * sizeof(SegRec));
+    const uint64_t n = h->n;
+    const uint64_t C_full = budget / (n * sizeof(SegRec));
     // a budget-derived depth stops at 4096 records (seconds of motion per
     // chunk for the densest workloads): small ensembles need no more
-    C = std::min<uint64_t>(C, 4096);
-    if (h->ext.chunk_records > 0) C = (uint64_t)h->ext.chunk_records;
+    uint64_t C = std::min<uint64_t>(C_full, 4096);
+    // Overflow pages for the particles that need more than C records in a
+    // chunk: a quarter of the budget when it limits the depth (three
+    // quarters as depth), else up to a quarter of the depth's records again,
+    // within the budget.  An explicit chunk depth gets no pool.
+    uint64_t pool = 0;
+    if (h->ext.chunk_records > 0) {
+        C = (uint64_t)h->ext.chunk_records;
+    } else if (C_full > C) {
+        pool = std::min<uint64_t>(C_full * n - C * n, C * n / 4);
+    } else {
+        C = std::max<uint64_t>(C_full * 3 / 4, 16);
+        pool = C_full > C ? (C_full - C) * n : 0;
+    }
     C = std::min<uint64_t>(std::max<uint64_t>(C, 16), 1u << 20);
     h->rec_cap = (uint32_t)C;
+    const uint32_t pages = (uint32_t)std::min<uint64_t>(pool / OVF_PAGE, 0xfffffff0u);
+    if (pages > 0) {
+        HIPCHK(h, hipMalloc(&h->d_ovf, (size_t)pages * OVF_PAGE * sizeof(SegRec)));
+        HIPCHK(h, hipMalloc(&h->d_ovf_link, sizeof(uint32_t) * (size_t)pages));
+        HIPCHK(h, hipMalloc(&h->d_ovf_first, sizeof(uint32_t) * (size_t)h->n));
+        h->ovf_pages = pages;
+    }
     HIPCHK(h, hipMalloc(&h->d_recs, (size_t)C * h->n * sizeof(SegRec)));
     HIPCHK(h, hipMalloc(&h->d_list[0], sizeof(uint32_t) * h->n));
     HIPCHK(h, hipMalloc(&h->d_list[1], sizeof(uint32_t) * h->n));
@@ -617,11 +641,20 @@ static int run_pipeline(st_handle* h, RunArgs R) {
         (integ == INT_NONE || spectral) ? nullptr : pick_spin(h->ext.variant, integ, h->batch, h->any_pulse);
     if (h->ext.graph_chunks > 0 && !h->ckpt_on && !h->ktime_on && !spectral)
         return run_pipeline_graph(h, A, spin, kbs);
+    if (!spectral && !h->ckpt_on && h->ovf_pages > 0) {
+        // the checkpointed (adjoint) and spectral paths keep the fixed layout
+        A.ovf = h->d_ovf;
+        A.ovf_first = h->d_ovf_first;
+        A.ovf_link = h->d_ovf_link;
+        A.ovf_head = h->d_small + 3;
+        A.ovf_pages = h->ovf_pages;
+    }
     while (n_list > 0) {
         A.list = h->d_list[cur];
         A.n_list = n_list;
         A.next_list = h->d_list[cur ^ 1];
-        // [0] spin queue, [1] next count, [2] failures, [4 + b] queue of spectral block b
+        // [0] spin queue, [1] next count, [2] failures, [3] overflow pages taken,
+        // [4 + b] queue of spectral block b
         HIPCHK(h, hipMemsetAsync(h->d_small, 0, 8 * sizeof(uint32_t), h->stream));
         if (h->ktime_on) HIPCHK(h, hipEventRecord(h->kev[0], h->stream));
         launch_kin(h, h->D, A, n_list, kbs);
@@ -1008,7 +1041,8 @@ ST_API void st_destroy(st_handle* h) {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     void* bufs[] = {h->state, h->d_targets, h->d_stop_at, h->d_work, h->d_small, h->d_trace, h->d_trace_count,
-                    h->d_recs, h->d_list[0], h->d_list[1], h->d_seg_count, h->d_backup, h->d_fail,
+                    h->d_recs, h->d_ovf, h->d_ovf_first, h->d_ovf_link, h->d_list[0], h->d_list[1],
+                    h->d_seg_count, h->d_backup, h->d_fail,
                     h->d_phys, h->d_cfg_off, h->d_cfg_seed, h->d_adj, h->d_replay, h->d_scratch, h->d_adj_recs,
                     h->spec.agg, h->spec.nsamp, h->d_spec, h->d_spec_n, h->d_spec_part};
     free_ckpts(h);

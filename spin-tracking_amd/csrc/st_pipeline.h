@@ -58,7 +58,32 @@ struct PipeArgs {
     // graph-captured chunks: the list length is read from here (set on the
     // device by the chunk head) instead of n_list; nullptr otherwise
     const uint32_t* n_list_dev;
+    // Overflow pages: the records j >= C of a particle that needs more than
+    // the chunk depth this chunk, in pages of OVF_PAGE records linked per
+    // particle, taken from a shared pool.  The number of segments per
+    // particle is heavy-tailed (a few bounce far more often than the mean),
+    // and without the pool those few would run a further chunk on their own,
+    // a handful of waves on an idle GPU.  ovf_pages == 0: no pool (the
+    // particle ends the chunk at C records).
+    char* ovf;               // ovf_pages * OVF_PAGE records
+    uint32_t* ovf_first;     // per particle: its first page this chunk
+    uint32_t* ovf_link;      // per page: the same particle's next page
+    uint32_t* ovf_head;      // pages taken this chunk (zeroed per chunk)
+    uint32_t ovf_pages;
 };
+
+constexpr uint32_t OVF_PAGE = 256;
+constexpr uint32_t OVF_NONE = 0xffffffffu;
+
+// Address of record j of particle pid, j taken in increasing order from 0 by
+// one lane (page: that lane's cursor, OVF_NONE before the first overflow
+// page; it is advanced here when j opens a page).
+__device__ __forceinline__ char* rec_at_seq(const PipeArgs& A, uint32_t pid, uint32_t j, uint32_t& page) {
+    if (j < A.C) return A.recs + ((uint64_t)pid * A.C + j) * sizeof(SegRec);
+    const uint32_t o = (j - A.C) % OVF_PAGE;
+    if (o == 0) page = (j == A.C) ? A.ovf_first[pid] : A.ovf_link[page];
+    return A.ovf + ((uint64_t)page * OVF_PAGE + o) * sizeof(SegRec);
+}
 
 __device__ __forceinline__ uint32_t list_len(const PipeArgs& A) { return A.n_list_dev ? *A.n_list_dev : A.n_list; }
 
@@ -157,9 +182,8 @@ __device__ __forceinline__ void kin_block(const RunArgs& R, const KinLane& L) {
                         L.K.n_coll, L.K.n_bounce, L.n_steps);
 }
 
-__device__ __forceinline__ void put_rec(char* recs, uint32_t C, uint32_t pid, uint32_t j, double t, const d3& p,
-                                        const d3& v, uint64_t flags) {
-    double* r = reinterpret_cast<double*>(recs + ((uint64_t)pid * C + j) * sizeof(SegRec));
+__device__ __forceinline__ void put_rec(char* at, double t, const d3& p, const d3& v, uint64_t flags) {
+    double* r = reinterpret_cast<double*>(at);
     r[0] = t; r[1] = p.x; r[2] = p.y; r[3] = p.z; r[4] = v.x; r[5] = v.y; r[6] = v.z;
     reinterpret_cast<uint64_t*>(r)[7] = flags;
 }
@@ -294,12 +318,22 @@ __global__ __launch_bounds__(256) void k_kin(const SimParams P, const DevState D
         const DevState B{A.backup, D.stride};
 #pragma unroll
         for (int f = 0; f < N_KIN_FIELDS; ++f) B.st<uint64_t>(KIN_FIELDS[f], o, D.ld<uint64_t>(KIN_FIELDS[f], o));
-        uint32_t j = 0;
-        while (j < A.C && !L.done) {
+        uint32_t j = 0, lim = A.C, page = OVF_NONE, rpage = OVF_NONE;
+        while (!L.done) {
+            if (j == lim) {
+                // chunk depth used up: continue on an overflow page, if one is left
+                if (A.ovf_pages == 0) break;
+                const uint32_t p = atomicAdd(A.ovf_head, 1u);
+                if (p >= A.ovf_pages) break;
+                if (page == OVF_NONE) A.ovf_first[L.pid] = p;
+                else A.ovf_link[page] = p;
+                page = p;
+                lim += OVF_PAGE;
+            }
             double rt;
             d3 rp, rv;
             const uint64_t fl = kin_step(P, F, A.R, A.integ, L, rt, rp, rv);
-            put_rec(A.recs, A.C, L.pid, j, rt, rp, rv, fl);
+            put_rec(rec_at_seq(A, L.pid, j, rpage), rt, rp, rv, fl);
             ++j;
             cnt[0] += (fl & R_EMPTY) ? 0 : 1;
         }
@@ -316,6 +350,7 @@ __global__ __launch_bounds__(256) void k_kin(const SimParams P, const DevState D
 // ---------------------------------------------------------------------------
 struct SpinLane {
     uint32_t pid, j, nrec;
+    uint32_t page;       // overflow-page cursor of the record reads
     uint32_t cfg;        // batched handles: the particle's configuration
     int32_t k;           // interval index of the next snapshot block
     bool stopped;
@@ -330,6 +365,7 @@ __device__ __forceinline__ void spin_load(const DevState& D, const PipeArgs& A, 
     L.pid = pid;
     L.cfg = BATCH ? (uint32_t)D.ld<uint64_t>(F_CFG, o) : 0u;
     L.j = 0;
+    L.page = OVF_NONE;
     L.nrec = A.seg_count[pid];
     const uint64_t ss = D.ld<uint64_t>(F_SSTATE, o);
     L.k = (int32_t)(uint32_t)ss;
@@ -353,12 +389,16 @@ __device__ __forceinline__ void spin_store(const DevState& D, const SpinLane& L)
     D.st<uint64_t>(F_SEG, o, D.ld<uint64_t>(F_SEG, o) + L.seg);
 }
 
-__device__ __forceinline__ SegRec get_rec(const char* recs, uint32_t C, uint32_t pid, uint32_t j) {
-    const double* r = reinterpret_cast<const double*>(recs + ((uint64_t)pid * C + j) * sizeof(SegRec));
+__device__ __forceinline__ SegRec load_rec(const char* at) {
+    const double* r = reinterpret_cast<const double*>(at);
     SegRec s;
     s.t = r[0]; s.px = r[1]; s.py = r[2]; s.pz = r[3]; s.vx = r[4]; s.vy = r[5]; s.vz = r[6];
     s.flags = reinterpret_cast<const uint64_t*>(r)[7];
     return s;
+}
+// record j of particle pid in the fixed layout (no overflow pages)
+__device__ __forceinline__ SegRec get_rec(const char* recs, uint32_t C, uint32_t pid, uint32_t j) {
+    return load_rec(recs + ((uint64_t)pid * C + j) * sizeof(SegRec));
 }
 
 __device__ __forceinline__ void spin_block(const RunArgs& R, const SpinLane& L) {
@@ -419,7 +459,7 @@ __device__ __forceinline__ void spin_body(const SimParams& P, const DevState& D,
                     }
                     break;
                 }
-                const SegRec rec = get_rec(A.recs, A.C, L.pid, L.j);
+                const SegRec rec = load_rec(rec_at_seq(A, L.pid, L.j, L.page));
                 cur_flags = (uint32_t)rec.flags;
                 if (rec.flags & R_FIRST) L.tempH = L.h;
                 if ((rec.flags & R_INTEGRATE) && !L.stopped) {

@@ -217,6 +217,45 @@ def test_he3_segment_replay(oracle, variant):
         assert worst < (1e-13 if variant == st.VARIANT_EXACT else 1e-11)
 
 
+@pytest.mark.parametrize("variant", [st.VARIANT_EXACT, st.VARIANT_FAST])
+def test_overflow_pages_match_deep_chunks(variant):
+    """Staging budgets that limit the chunk depth, for 1024 neutrons over 2 s
+    (~150 records each).  4 MiB: depth 48 and a pool of 64 overflow pages of
+    256 records -- the first particles to reach the depth take the pages
+    until the pool runs out, the rest continue in later chunks.  16 MiB:
+    depth 192 and 256 pages -- enough for every particle above the depth, so
+    one chunk does.  Per-particle results are those of one deep chunk
+    without a pool, bit for bit."""
+    opt = configs.neutron_cell(n=1024, tf=2.0, ioutInt=0.5)
+    opt.integratorType = 2 if variant == st.VARIANT_FAST else 0
+    targets = st.output_times(opt)
+
+    def run(**kw):
+        p = st.Particle(opt, variant=variant, **kw)
+        p.initParticles()
+        p.kernel_timing(True)
+        p.run(targets)
+        out = p.snapshot(), p.get_state(), p.particle_counters(), p.kernel_times()
+        p.close()
+        return out
+
+    g_deep, s_deep, c_deep, k_deep = run(chunk_records=4096)
+    assert k_deep["kin_launches"] == 1
+    assert np.mean(c_deep["segments"] > 48) > 0.9   # most particles need more than the small depth
+    for mb in (4, 16):
+        g, s_, c, k = run(seg_buffer_mb=mb)
+        assert s_.tobytes() == s_deep.tobytes(), mb
+        for name in c_deep:
+            assert np.array_equal(c[name], c_deep[name]), (mb, name)
+        for name in g_deep:
+            assert np.asarray(g[name]).tobytes() == np.asarray(g_deep[name]).tobytes(), (mb, name)
+        print(f"{mb} MiB: kinematics launches {k['kin_launches']}", flush=True)
+        if mb == 16 and c_deep["segments"].max() > 192:
+            # the same depth without a pool needs a second chunk; with it, one
+            assert k["kin_launches"] == 1
+            assert run(chunk_records=192)[3]["kin_launches"] >= 2
+
+
 @pytest.mark.parametrize("engine", [PIPE, FUSED])
 def test_integrator_failure_nmax(oracle, engine):
     """nmax = 4: segments needing more than 5 trial steps stop the particle
