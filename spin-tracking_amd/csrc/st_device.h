@@ -551,23 +551,31 @@ struct Poly {
     int32_t paxis, pon;
 };
 
-// Taylor coefficients in x^2 of cos x (row 0: (-1)^k / (2k)!) and of
-// sin(x)/x (row 1: (-1)^k / (2k+1)!), k = 0..10.  The device reads them from
+// Series coefficients: row 0, cos x in x^2 ((-1)^k / (2k)!); row 1,
+// sin(x)/x in x^2 ((-1)^k / (2k+1)!); row 2, atanh(s)/s in s^2
+// (1 / (2k+1)); row 3, e^r in r (1 / k!).  The device reads them from
 // constant memory at uniform offsets, so every Horner step is one FMA with a
 // scalar-register operand: written as literals, each coefficient that is
 // not an inline constant costs two extra vector moves per use (a 64-bit
 // literal cannot be an FMA operand, and the accumulating form wants it in
 // vector registers).
 #define ST_TAYLOR_COS 1.0, -1.0 / 2, 1.0 / 24, -1.0 / 720, 1.0 / 40320, -1.0 / 3628800, 1.0 / 479001600, \
-    -1.0 / 87178291200.0, 1.0 / 20922789888000.0, -1.0 / 6402373705728000.0, 1.0 / 2432902008176640000.0
+    -1.0 / 87178291200.0, 1.0 / 20922789888000.0, -1.0 / 6402373705728000.0, 1.0 / 2432902008176640000.0, 0.0, 0.0, 0.0
 #define ST_TAYLOR_SINC 1.0, -1.0 / 6, 1.0 / 120, -1.0 / 5040, 1.0 / 362880, -1.0 / 39916800, 1.0 / 6227020800.0, \
-    -1.0 / 1307674368000.0, 1.0 / 355687428096000.0, -1.0 / 121645100408832000.0, 1.0 / 51090942171709440000.0
+    -1.0 / 1307674368000.0, 1.0 / 355687428096000.0, -1.0 / 121645100408832000.0, 1.0 / 51090942171709440000.0, \
+    0.0, 0.0, 0.0
+#define ST_TAYLOR_ATANH 1.0, 1.0 / 3, 1.0 / 5, 1.0 / 7, 1.0 / 9, 1.0 / 11, 1.0 / 13, 1.0 / 15, 1.0 / 17, 1.0 / 19, \
+    1.0 / 21, 0.0, 0.0, 0.0
+#define ST_TAYLOR_EXP 1.0, 1.0, 1.0 / 2, 1.0 / 6, 1.0 / 24, 1.0 / 120, 1.0 / 720, 1.0 / 5040, 1.0 / 40320, \
+    1.0 / 362880, 1.0 / 3628800, 1.0 / 39916800, 1.0 / 479001600, 1.0 / 6227020800.0
 #if defined(__HIP_DEVICE_COMPILE__)   // device-only: no host shadow to register
-__constant__ double TAYLOR_DEV[2][11] = {{ST_TAYLOR_COS}, {ST_TAYLOR_SINC}};
+__constant__ double TAYLOR_DEV[4][14] = {{ST_TAYLOR_COS}, {ST_TAYLOR_SINC}, {ST_TAYLOR_ATANH}, {ST_TAYLOR_EXP}};
 #endif
-constexpr double TAYLOR_HOST[2][11] = {{ST_TAYLOR_COS}, {ST_TAYLOR_SINC}};
+constexpr double TAYLOR_HOST[4][14] = {{ST_TAYLOR_COS}, {ST_TAYLOR_SINC}, {ST_TAYLOR_ATANH}, {ST_TAYLOR_EXP}};
 #undef ST_TAYLOR_COS
 #undef ST_TAYLOR_SINC
+#undef ST_TAYLOR_ATANH
+#undef ST_TAYLOR_EXP
 
 // sum_{k<N} TAYLOR[row][k] x2^k by Horner, one FMA per term
 template <int ROW, int N>
@@ -607,6 +615,46 @@ ST_HD __forceinline__ void small_sincos(double th, double& s, double& c) {
     const double t2 = th * th;
     c = taylor_sum<0, 8>(t2);
     s = th * taylor_sum<1, 8>(t2);
+}
+
+// Natural log and exp for the fast variant's step-size controllers, whose
+// factor (tol/err)^a (prev)^b only steers the step: a few ulp suffice, at a
+// third of the instructions of the library's correctly rounded forms.
+// ctl_log(x), x finite and > 0: x = m 2^e with m in [sqrt(1/2), sqrt(2)),
+// ln x = e ln 2 + 2 atanh(s), s = (m - 1)/(m + 1), |s| < 0.1716, by the
+// series to s^21 (first omitted term below 2e-17 of the sum).  ctl_exp(y),
+// |y| < 700: y = n ln 2 + r (n = rint(y / ln 2), ln 2 in two parts so r is
+// exact to ~1e-19), |r| <= 0.347, e^r by its Taylor series to r^13 (first
+// omitted term below 5e-18), scaled by 2^n.
+constexpr double LN2_HI = 6.93147180369123816490e-01;   // ln 2 to 32 bits (n * LN2_HI exact)
+constexpr double LN2_LO = 1.90821492927058770002e-10;   // ln 2 - LN2_HI
+constexpr double INV_LN2 = 1.44269504088896338700e+00;
+ST_HD __forceinline__ double ctl_log(double x) {
+#pragma clang fp contract(fast)
+    if (!(x > 0.0 && x <= DBL_MAX_D)) return log(x);   // 0, inf, NaN, negative: the library's values
+    int e;
+    double m = frexp(x, &e);   // [0.5, 1)
+    if (m < 0.70710678118654752440) {
+        m *= 2.0;
+        e -= 1;
+    }
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double r0 = __builtin_amdgcn_rcp(m + 1.0);
+    const double r = __builtin_fma(r0, __builtin_fma(-(m + 1.0), r0, 1.0), r0);
+    const double sv = (m - 1.0) * r;
+#else
+    const double sv = (m - 1.0) / (m + 1.0);
+#endif
+    const double p = taylor_sum<2, 11>(sv * sv);
+    return (double)e * LN2_HI + ((double)e * LN2_LO + 2.0 * sv * p);
+}
+ST_HD __forceinline__ double ctl_exp(double y) {
+#pragma clang fp contract(fast)
+    // (n clamped so the conversion is defined; beyond +-2000 the result is
+    // 0 or inf either way, and a NaN y gives a NaN r)
+    const double n = fmin(fmax(rint(y * INV_LN2), -2000.0), 2000.0);
+    const double r = (y - n * LN2_HI) - n * LN2_LO;
+    return ldexp(taylor_sum<3, 14>(r), (int)n);
 }
 
 // gamma*B along a segment plus the pulse value pv (already scaled like the

@@ -377,10 +377,11 @@ static const void* fused_symbol(int variant, int integ, bool pulse) {
 // have no pulse; the other kernels evaluate the pulse switch at run time.
 // The fast DOP853 kernels run at occupancy 3 (the pulse form spilling ~90 B
 // per lane to do so), except the batched pulse form, whose per-lane pulse
-// fields make the spills cost more than the third wave gains.
+// fields make the spills cost more than the third wave gains; so does the
+// fast MagnusCFET kernel, which sits at the 168-register line.
 template <int VAR, int INTEG, bool BATCH, bool PULSE>
 constexpr bool use_o3() {
-    return VAR == VAR_FAST && INTEG == INT_DOP853;
+    return VAR == VAR_FAST && (INTEG == INT_DOP853 || INTEG == INT_MAGNUS);
 }
 template <int VAR, int INTEG, bool BATCH, bool PULSE>
 static hipError_t launch_spin(st_handle* h, dim3 grid, const PipeArgs& A) {
@@ -415,7 +416,9 @@ template <int VAR, bool BATCH>
 static const void* spin_symbol_v(int integ, bool pulse) {
     switch (integ) {
     case INT_RK45: return (const void*)&k_spin<VAR, INT_RK45, BATCH, true>;
-    case INT_MAGNUS: return (const void*)&k_spin<VAR, INT_MAGNUS, BATCH, true>;
+    case INT_MAGNUS:
+        return use_o3<VAR, INT_MAGNUS, BATCH, true>() ? (const void*)&k_spin_o3<VAR, INT_MAGNUS, BATCH, true>
+                                                      : (const void*)&k_spin<VAR, INT_MAGNUS, BATCH, true>;
     case INT_RK45Q: return (const void*)&k_spin<VAR, INT_RK45Q, BATCH, true>;
     case INT_RKF45: return (const void*)&k_spin<VAR, INT_RKF45, BATCH, true>;
     case INT_RKF45Q: return (const void*)&k_spin<VAR, INT_RKF45Q, BATCH, true>;

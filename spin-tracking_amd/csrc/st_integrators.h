@@ -574,22 +574,28 @@ ST_HD inline int rk_trial(const SimParams& P, const Phys& Fp, RkState& R, d3& y,
     ST_V3(te, h * (CT1 * k1.x + CT2 * k2.x + CT3 * k3.x + CT4 * k4.x + CT5 * k5.x + CT6 * k6.x),
           h * (CT1 * k1.y + CT2 * k2.y + CT3 * k3.y + CT4 * k4.y + CT5 * k5.y + CT6 * k6.y),
           h * (CT1 * k1.z + CT2 * k2.z + CT3 * k3.z + CT4 * k4.z + CT5 * k5.z + CT6 * k6.z));
-    double error = len3(te);
-    error = std_max(error, 1.0E-16);
     const double tol = P.rtol;
-    const double ratio = tol / error;
     double q_;
+    bool small;   // error < 2 tol
     if constexpr (VAR == VAR_FAST) {
         // ratio^0.1 * prev^(-0.4/7) as one exp of a sum of logs, the previous
-        // trial's log carried over: one log and one exp instead of two pows
-        const double lr = log(ratio);
-        q_ = exp((0.7 / 7.0) * lr + (-0.4 / 7.0) * R.prev_lr);
+        // trial's log carried over, from the squared norm (no square root,
+        // no division): one ctl_log and one ctl_exp instead of two pows
+        const double e2 = std_max(te.x * te.x + te.y * te.y + te.z * te.z, 1.0E-32);
+        const double lr = 0.5 * ctl_log(tol * tol * fast_rcp(e2));
+        q_ = ctl_exp((0.7 / 7.0) * lr + (-0.4 / 7.0) * R.prev_lr);
         R.prev_lr = lr;
+        small = e2 < 4.0 * tol * tol;
     } else {
+        double error = len3(te);
+        error = std_max(error, 1.0E-16);
+        const double ratio = tol / error;
         q_ = pow(ratio, 0.7 / 7.0) * pow(R.prev_ratio, -0.4 / 7.0);
+        small = error < 2 * tol;
+        R.prev_ratio = ratio;
     }
     q_ = std_min(q_, 4.0);
-    if ((q_ > 0.81 && error < 2 * tol) || P.fixed) {
+    if ((q_ > 0.81 && small) || P.fixed) {
         nacc++;
         y = ws;
         R.t += h;
@@ -602,7 +608,6 @@ ST_HD inline int rk_trial(const SimParams& P, const Phys& Fp, RkState& R, d3& y,
         if (R.stop) R.stop = false;
     }
     if (!P.fixed) h = h * q_;
-    R.prev_ratio = ratio;
     return 0;
 }
 
@@ -731,6 +736,40 @@ ST_HD __forceinline__ d3 rodriguez_v(const d3& k, const d3& v) {
     }
     return rodriguez(k, v);
 }
+// Fast quaternion forms for chains of rotations.  rot_quat_fast(k) is the
+// unit quaternion of rodriguez(k, .) -- a rotation by theta = |k| about
+// -k/|k|: (cos(theta/2), -(k/2) sin(theta/2)/(theta/2)) -- from the same
+// series and doublings as rodriguez_v (one doubling fewer: the half angle is
+// what a quaternion holds); the vectors the exact form takes go through
+// rodriguez_quat(k, 1), which keeps the reference's NaN for a zero or
+// non-finite k.  A chain of rotations is then composed with 16-operation
+// quaternion products and applied to the vector once (q v q* as
+// v + w t + u x t, t = 2 u x v), in place of a 30-operation vector rotation
+// per factor; the same rotations, rounded differently.
+ST_HD __forceinline__ Quat rot_quat_fast(const d3& k) {
+#pragma clang fp contract(fast)
+    const double t2 = k.x * k.x + k.y * k.y + k.z * k.z;
+    if (t2 > 0.0 && t2 <= FAST_ROT_MAX_X2) {
+        double ch, sh;
+        cos_sinc_doubled<1>(0.25 * t2, ch, sh);
+        const double s = -0.5 * sh;
+        return Quat{ch, k.x * s, k.y * s, k.z * s};
+    }
+    return rodriguez_quat(k, 1.0);
+}
+ST_HD __forceinline__ Quat q_mult_fast(const Quat& a, const Quat& b) {
+#pragma clang fp contract(fast)
+    return Quat{a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z, a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y,
+                a.w * b.y + a.y * b.w + a.z * b.x - a.x * b.z, a.w * b.z + a.z * b.w + a.x * b.y - a.y * b.x};
+}
+ST_HD __forceinline__ d3 qrot_fast(const Quat& q, const d3& v) {
+#pragma clang fp contract(fast)
+    const double tx = 2.0 * (q.y * v.z - q.z * v.y), ty = 2.0 * (q.z * v.x - q.x * v.z),
+                 tz = 2.0 * (q.x * v.y - q.y * v.x);
+    return d3{v.x + q.w * tx + (q.y * tz - q.z * ty), v.y + q.w * ty + (q.z * tx - q.x * tz),
+              v.z + q.w * tz + (q.x * ty - q.y * tx)};
+}
+
 // the quaternion of a rotation by |k| dt: half angle p = |k| dt / 2,
 // (cos p, -k (dt/2) sin(p)/p)
 template <int VAR>
@@ -861,24 +900,42 @@ ST_HD __forceinline__ int bounds_v2(const SimParams& P, RkState& R, double& h) {
     return 0;
 }
 
+// The error argument of control_v2: the norm of the error vector d (exact
+// variant), or its square (fast variant, which needs no square root).
+template <int VAR>
+ST_HD __forceinline__ double err_arg(const d3& d) {
+    if constexpr (VAR == VAR_FAST) {
+#pragma clang fp contract(fast)
+        return d.x * d.x + d.y * d.y + d.z * d.z;
+    } else {
+        return len3(d);
+    }
+}
+
 // Controller and accept test (e.g. :400-424); 1 when the call ends.  The
-// fast variant forms the step factor from logs, as in rk_trial.
+// fast variant forms the step factor from logs of the squared norm, as in
+// rk_trial.
 template <int VAR>
 ST_HD __forceinline__ int control_v2(const SimParams& P, RkState& R, d3& y, const d3& ynew, double& h,
-                                     double error, uint32_t& nacc, uint32_t& nrej) {
-    error = std_max(error, 1.0E-16);
+                                     double err_in, uint32_t& nacc, uint32_t& nrej) {
     const double tol = P.rtol;
-    const double ratio = tol / error;
     double q_;
+    bool small;   // error < 2 tol
     if constexpr (VAR == VAR_FAST) {
-        const double lr = log(ratio);
-        q_ = exp((0.7 / 7.0) * lr + (-0.4 / 7.0) * R.prev_lr);
+        const double e2 = std_max(err_in, 1.0E-32);
+        const double lr = 0.5 * ctl_log(tol * tol * fast_rcp(e2));
+        q_ = ctl_exp((0.7 / 7.0) * lr + (-0.4 / 7.0) * R.prev_lr);
         R.prev_lr = lr;
+        small = e2 < 4.0 * tol * tol;
     } else {
+        const double error = std_max(err_in, 1.0E-16);
+        const double ratio = tol / error;
         q_ = pow(ratio, 0.7 / 7.0) * pow(R.prev_ratio, -0.4 / 7.0);
+        small = error < 2 * tol;
+        R.prev_ratio = ratio;
     }
     q_ = std_min(q_, 4.0);
-    if ((q_ > 0.81 && error < 2 * tol) || P.fixed) {
+    if ((q_ > 0.81 && small) || P.fixed) {
         nacc++;
         y = ynew;
         R.t += h;
@@ -891,7 +948,6 @@ ST_HD __forceinline__ int control_v2(const SimParams& P, RkState& R, d3& y, cons
         if (R.stop) R.stop = false;
     }
     if (!P.fixed) h = h * q_;
-    R.prev_ratio = ratio;
     return 0;
 }
 
@@ -927,20 +983,20 @@ ST_HD __forceinline__ int quat_trial_fast(const SimParams& P, RkState& R, d3& y,
             const double t2 = k.x * k.x + k.y * k.y + k.z * k.z;
             degenerate = degenerate || !(t2 > 0.0 && t2 <= DBL_MAX_D);
         }
-        if (ww != 0.0) qw = q_mult(rodriguez_quat_v<VAR_FAST>(k, h * ww), qw);
+        if (ww != 0.0) qw = q_mult_fast(rodriguez_quat_v<VAR_FAST>(k, h * ww), qw);
         if (i < 5) {
             if constexpr (SHARED) qe = qw;
-            else if (we != 0.0) qe = q_mult(rodriguez_quat_v<VAR_FAST>(k, h * we), qe);
+            else if (we != 0.0) qe = q_mult_fast(rodriguez_quat_v<VAR_FAST>(k, h * we), qe);
         }
     }
     if (degenerate) {
         const double nan = __builtin_nan("");
         qw = qe = Quat{nan, nan, nan, nan};
     }
-    const d3 ws = qv_mult(qw, y);
-    const d3 te = qv_mult(qe, y);
+    const d3 ws = qrot_fast(qw, y);
+    const d3 te = qrot_fast(qe, y);
     const d3 d{ws.x - te.x, ws.y - te.y, ws.z - te.z};
-    return control_v2<VAR_FAST>(P, R, y, ws, h, len3(d), nacc, nrej);
+    return control_v2<VAR_FAST>(P, R, y, ws, h, err_arg<VAR_FAST>(d), nacc, nrej);
 }
 
 template <int VAR, int INTEG>
@@ -998,7 +1054,7 @@ ST_HD inline int rk2_trial(const SimParams& P, const Phys& Fp, RkState& R, d3& y
         ST_V3(te, h * (R1 * k1.x + R2 * k2.x + R3 * k3.x + R4 * k4.x + R5 * k5.x + R6 * k6.x),
               h * (R1 * k1.y + R2 * k2.y + R3 * k3.y + R4 * k4.y + R5 * k5.y + R6 * k6.y),
               h * (R1 * k1.z + R2 * k2.z + R3 * k3.z + R4 * k4.z + R5 * k5.z + R6 * k6.z));
-        return control_v2<VAR>(P, R, y, ws, h, len3(te), nacc, nrej);
+        return control_v2<VAR>(P, R, y, ws, h, err_arg<VAR>(te), nacc, nrej);
     } else if constexpr (INTEG == INT_RK45Q || INTEG == INT_RKF45Q) {
         // k_i = gamma*B(t + A_i h); the step rotates y by the product of
         // R(k_i, h W_i), k1 first; the comparison solution drops the sixth
@@ -1017,7 +1073,7 @@ ST_HD inline int rk2_trial(const SimParams& P, const Phys& Fp, RkState& R, d3& y
         const d3 ws = qv_mult(qw, y);
         const d3 te = qv_mult(qe, y);
         const d3 d{ws.x - te.x, ws.y - te.y, ws.z - te.z};
-        return control_v2<VAR>(P, R, y, ws, h, len3(d), nacc, nrej);
+        return control_v2<VAR>(P, R, y, ws, h, err_arg<VAR>(d), nacc, nrej);
     } else {
         static_assert(INTEG == INT_MAGNUS, "integrator type");
         using namespace cfet;
@@ -1053,14 +1109,16 @@ ST_HD inline int rk2_trial(const SimParams& P, const Phys& Fp, RkState& R, d3& y
                 }
                 return k;
             };
-            d3 y8 = y, y6 = y;
+            // both chains as quaternion products (the first factor acts first)
+            Quat q8{1.0, 0.0, 0.0, 0.0}, q6{1.0, 0.0, 0.0, 0.0};
 #pragma unroll 1
             for (int r = 0; r < 11; ++r) {
-                y8 = rodriguez_v<VAR>(expo(r), y8);
-                if (r < 5) y6 = rodriguez_v<VAR>(expo(11 + r), y6);
+                q8 = q_mult_fast(rot_quat_fast(expo(r)), q8);
+                if (r < 5) q6 = q_mult_fast(rot_quat_fast(expo(11 + r)), q6);
             }
+            const d3 y8 = qrot_fast(q8, y), y6 = qrot_fast(q6, y);
             const d3 d{y8.x - y6.x, y8.y - y6.y, y8.z - y6.z};
-            return control_v2<VAR>(P, R, y, y8, h, len3(d), nacc, nrej);
+            return control_v2<VAR>(P, R, y, y8, h, err_arg<VAR>(d), nacc, nrej);
         }
         const d3 B[5] = {W(t + X1 * h), W(t + X2 * h), W(t + X3 * h), W(t + X4 * h), W(t + X5 * h)};
         // order 8: k11 (row 1 reversed) acts first ... k6 (row 6) ... k1
@@ -1080,7 +1138,7 @@ ST_HD inline int rk2_trial(const SimParams& P, const Phys& Fp, RkState& R, d3& y
             }
         }
         const d3 d{y8.x - y6.x, y8.y - y6.y, y8.z - y6.z};
-        return control_v2<VAR>(P, R, y, y8, h, len3(d), nacc, nrej);
+        return control_v2<VAR>(P, R, y, y8, h, err_arg<VAR>(d), nacc, nrej);
     }
 }
 

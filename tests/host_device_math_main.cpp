@@ -11,6 +11,8 @@
 //     and diffuse walls, Maxwell-Boltzmann and isotropic gas collisions, the
 //     interval cap, +-0, +-inf, NaN and tiny velocity components, positions on
 //     the walls, an apex exactly at the ceiling.
+// The fast variant's step-size log and exp (ctl_log / ctl_exp) are checked
+// against the C library to 1e-15 relative.
 // Prints "cases N mismatches M" and exits non-zero on any mismatch.
 //
 // Diffuse re-emission takes sin and cos of two random angles: the device code
@@ -195,6 +197,35 @@ int main(int argc, char** argv) {
                 check(same_rng(drng, orng), "rng after event", i);
                 ++cases;
             }
+    // ---- the fast controllers' log and exp (st_device.h ctl_log / ctl_exp)
+    // against the C library: relative error within 1e-15 ----
+    {
+        std::mt19937_64 g2(5);
+        std::uniform_real_distribution<double> ue(-700.0, 700.0), uy(-700.0, 700.0);
+        double worst_log = 0.0, worst_exp = 0.0;
+        const double xs[] = {1.0, 2.0, 0.5, 0.7071067811865476, 0.7071067811865475, 1.4142135623730951,
+                             1.0000000000000002, 0.9999999999999999, 1e-300, 4.9e-324, 1.7976931348623157e308};
+        const double ys[] = {0.0, -0.0, 1e-300, -1e-300, 0.5, -0.5, 0.34657359027997264, 700.0, -700.0, 709.0,
+                             -708.0};
+        for (long i = 0; i < n + 22; ++i) {
+            const double x = i < 11 ? xs[i] : std::exp(ue(g2));
+            const double y = i < 11 ? ys[i] : uy(g2);
+            const double l = st::ctl_log(x), lr = std::log(x);
+            const double e = st::ctl_exp(y), er = std::exp(y);
+            const double dl = lr == 0.0 ? std::fabs(l) : std::fabs(l - lr) / std::fabs(lr);
+            const double de = std::fabs(e - er) / er;
+            worst_log = std::fmax(worst_log, dl);
+            worst_exp = std::fmax(worst_exp, de);
+            check(dl <= 1e-15, "ctl_log", i);
+            check(de <= 1e-15, "ctl_exp", i);
+            ++cases;
+        }
+        check(std::isnan(st::ctl_log(NAN)) && std::isinf(st::ctl_log(0.0)) && std::isinf(st::ctl_log(INFINITY)),
+              "ctl_log special", 0);
+        check(std::isnan(st::ctl_exp(NAN)) && st::ctl_exp(-1e6) == 0.0 && std::isinf(st::ctl_exp(1e6)),
+              "ctl_exp special", 0);
+        printf("ctl_log worst relative error %.3g, ctl_exp %.3g\n", worst_log, worst_exp);
+    }
     printf("diffuse events %ld (last-bit differences %ld)\n", g_diffuse, g_diffuse_inexact);
     printf("gas jobs %ld\n", g_gas_jobs);
     printf("cases %ld mismatches %ld\n", cases, g_bad);
