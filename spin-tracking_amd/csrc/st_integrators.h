@@ -698,13 +698,29 @@ ST_HD __forceinline__ void half_series(double x2, double& c, double& s) {
 constexpr double FAST_ROT_MAX_X2 = 0x1p60;   // at most 31 doublings
 // cos x and sin(x)/x for x^2 = X2 in (0, 2^60]: the series after d >= D0
 // halvings, d the fewest that bring the square to 1 or below
-template <int D0>
+// (VOTE: the wave decides once, with a scalar branch, whether any of its
+// lanes needs more than D0 halvings, so the common case pays one multiply
+// for the scaling; the branch waits for the compare, which costs more than
+// it saves in the MagnusCFET chain and less in the quaternion trials.)
+template <int D0, bool VOTE>
 ST_HD __forceinline__ void cos_sinc_doubled(double X2, double& c, double& s) {
 #pragma clang fp contract(fast)
-    int E;
-    (void)frexp(X2, &E);   // X2 < 2^E, so X2 / 4^d < 1 once 2d >= E
-    const int d = E > 2 * D0 ? (E + 1) >> 1 : D0;
-    double x2 = ldexp(X2, -2 * d);
+    constexpr double SCALE = D0 == 0 ? 1.0 : D0 == 1 ? 0.25 : 0.0625;   // 4^-D0
+    static_assert(D0 >= 0 && D0 <= 2, "D0");
+    const bool extra = X2 > 1.0 / SCALE;
+#if defined(__HIP_DEVICE_COMPILE__)
+    const bool wave_extra = VOTE ? __any(extra) : true;
+#else
+    const bool wave_extra = VOTE ? extra : true;
+#endif
+    int d = D0;
+    double x2 = X2 * SCALE;
+    if (wave_extra) {
+        int E;
+        (void)frexp(X2, &E);   // X2 < 2^E, so X2 / 4^d < 1 once 2d >= E
+        d = E > 2 * D0 ? (E + 1) >> 1 : D0;
+        x2 = ldexp(X2, -2 * d);
+    }
     half_series(x2, c, s);
     auto twice = [&]() {
         const double s2 = s * s;
@@ -712,7 +728,8 @@ ST_HD __forceinline__ void cos_sinc_doubled(double X2, double& c, double& s) {
         c = 1.0 - (2.0 * x2) * s2;
         x2 *= 4.0;
     };
-    for (int i = d; i > D0; --i) twice();
+    if (wave_extra)
+        for (int i = d; i > D0; --i) twice();
 #pragma unroll
     for (int i = 0; i < D0; ++i) twice();
 }
@@ -725,7 +742,7 @@ ST_HD __forceinline__ d3 rodriguez_v(const d3& k, const d3& v) {
             // cos(theta/2) and sin(theta/2) / (theta/2) from theta/4 (or
             // below), then the last doubling to theta
             double ch, sh;
-            cos_sinc_doubled<1>(0.25 * t2, ch, sh);
+            cos_sinc_doubled<1, false>(0.25 * t2, ch, sh);
             const double sa = sh * ch;           // sin(theta) / theta
             const double f1 = 0.5 * (sh * sh);   // (1 - cos theta) / theta^2
             const double c = 1.0 - t2 * f1;      // cos theta
@@ -751,7 +768,7 @@ ST_HD __forceinline__ Quat rot_quat_fast(const d3& k) {
     const double t2 = k.x * k.x + k.y * k.y + k.z * k.z;
     if (t2 > 0.0 && t2 <= FAST_ROT_MAX_X2) {
         double ch, sh;
-        cos_sinc_doubled<1>(0.25 * t2, ch, sh);
+        cos_sinc_doubled<1, false>(0.25 * t2, ch, sh);
         const double s = -0.5 * sh;
         return Quat{ch, k.x * s, k.y * s, k.z * s};
     }
@@ -780,7 +797,7 @@ ST_HD __forceinline__ Quat rodriguez_quat_v(const d3& k, double dt) {
         const double p2 = (k.x * k.x + k.y * k.y + k.z * k.z) * (hd * hd);
         if (p2 > 0.0 && p2 <= FAST_ROT_MAX_X2) {
             double c, sp;
-            cos_sinc_doubled<0>(p2, c, sp);
+            cos_sinc_doubled<0, true>(p2, c, sp);
             const double s = -(hd * sp);
             return Quat{c, k.x * s, k.y * s, k.z * s};
         }
