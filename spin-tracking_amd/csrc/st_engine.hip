@@ -488,20 +488,14 @@ static int ensure_pipeline(st_handle* h) {
     // a budget-derived depth stops at 4096 records (seconds of motion per
     // chunk for the densest workloads): small ensembles need no more
     uint64_t C = std::min<uint64_t>(C_full, 4096);
-    // Overflow pages for the particles that need more than C records in a
-    // chunk: a quarter of the budget when it limits the depth (three
-    // quarters as depth), else up to a quarter of the depth's records again,
-    // within the budget.  An explicit chunk depth gets no pool.
+    // Overflow pages for the particles near their end when they reach the
+    // depth (k_kin's rule): up to a quarter of the depth's records, on top
+    // of the budget by at most a quarter of it.  An explicit chunk depth
+    // gets no pool.
     uint64_t pool = 0;
-    if (h->ext.chunk_records > 0) {
-        C = (uint64_t)h->ext.chunk_records;
-    } else if (C_full > C) {
-        pool = std::min<uint64_t>(C_full * n - C * n, C * n / 4);
-    } else {
-        C = std::max<uint64_t>(C_full * 3 / 4, 16);
-        pool = C_full > C ? (C_full - C) * n : 0;
-    }
+    if (h->ext.chunk_records > 0) C = (uint64_t)h->ext.chunk_records;
     C = std::min<uint64_t>(std::max<uint64_t>(C, 16), 1u << 20);
+    if (h->ext.chunk_records == 0) pool = std::min<uint64_t>(C * n / 4, budget / 4 / sizeof(SegRec));
     h->rec_cap = (uint32_t)C;
     const uint32_t pages = (uint32_t)std::min<uint64_t>(pool / OVF_PAGE, 0xfffffff0u);
     if (pages > 0) {

@@ -5,6 +5,8 @@ Knobs (all in st_ext):
   chunk: the depth of the segment-record buffer in HBM (records x particles
   x 64 B).  Deeper chunks mean fewer kernel pairs and host synchronisations;
   shallower ones keep the buffer small and shorten the spin kernel's tail.
+  0 is the library default: the depth from the staging budget plus overflow
+  pages for the particles near their end.
 * ``block_size`` -- lanes per workgroup of the kinematics kernel.
 
 ``tune`` times a short probe (the first output interval(s) of the real
@@ -22,7 +24,8 @@ from . import _lib as L
 from .options import output_times
 from .particle import Particle
 
-DEFAULT_CHUNKS = (256, 512, 1024)
+# 0: the library's default staging (budget-derived depth plus overflow pages)
+DEFAULT_CHUNKS = (0, 256, 512, 1024)
 DEFAULT_BLOCKS = (128, 256)
 
 

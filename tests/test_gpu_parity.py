@@ -220,12 +220,13 @@ def test_he3_segment_replay(oracle, variant):
 @pytest.mark.parametrize("variant", [st.VARIANT_EXACT, st.VARIANT_FAST])
 def test_overflow_pages_match_deep_chunks(variant):
     """Staging budgets that limit the chunk depth, for 1024 neutrons over 2 s
-    (~150 records each).  4 MiB: depth 48 and a pool of 64 overflow pages of
-    256 records -- the first particles to reach the depth take the pages
-    until the pool runs out, the rest continue in later chunks.  16 MiB:
-    depth 192 and 256 pages -- enough for every particle above the depth, so
-    one chunk does.  Per-particle results are those of one deep chunk
-    without a pool, bit for bit."""
+    (~150 records each).  7 MiB: depth 112 and a pool of 112 overflow pages
+    of 256 records -- the particles that reach the depth with less than
+    twice the chunk's time left to go take pages until the pool runs out, the
+    rest continue in later chunks.  12 MiB: depth 192 and 192 pages, enough for
+    every particle that qualifies.  Per-particle results are those of one
+    deep chunk without a pool, bit for bit, in no more chunks than the same
+    depth without a pool."""
     opt = configs.neutron_cell(n=1024, tf=2.0, ioutInt=0.5)
     opt.integratorType = 2 if variant == st.VARIANT_FAST else 0
     targets = st.output_times(opt)
@@ -241,8 +242,8 @@ def test_overflow_pages_match_deep_chunks(variant):
 
     g_deep, s_deep, c_deep, k_deep = run(chunk_records=4096)
     assert k_deep["kin_launches"] == 1
-    assert np.mean(c_deep["segments"] > 48) > 0.9   # most particles need more than the small depth
-    for mb in (4, 16):
+    assert np.mean(c_deep["segments"] > 112) > 0.5   # many particles need more than the small depth
+    for mb in (7, 12):
         g, s_, c, k = run(seg_buffer_mb=mb)
         assert s_.tobytes() == s_deep.tobytes(), mb
         for name in c_deep:
@@ -250,10 +251,10 @@ def test_overflow_pages_match_deep_chunks(variant):
         for name in g_deep:
             assert np.asarray(g[name]).tobytes() == np.asarray(g_deep[name]).tobytes(), (mb, name)
         print(f"{mb} MiB: kinematics launches {k['kin_launches']}", flush=True)
-        if mb == 16 and c_deep["segments"].max() > 192:
-            # the same depth without a pool needs a second chunk; with it, one
-            assert k["kin_launches"] == 1
-            assert run(chunk_records=192)[3]["kin_launches"] >= 2
+        # the same depth without a pool needs at least as many chunks
+        same = run(chunk_records=112 if mb == 7 else 192)[3]["kin_launches"]
+        print(f"   same depth without a pool: {same}", flush=True)
+        assert k["kin_launches"] <= same
 
 
 @pytest.mark.parametrize("engine", [PIPE, FUSED])
