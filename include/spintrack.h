@@ -87,9 +87,14 @@ typedef struct st_ext {
                                fused per-interval kernel; multiples of 64 up to 256; 0 = 256 */
     int32_t engine;         /* ST_ENGINE_* */
     int32_t seg_buffer_mb;  /* HBM budget of the pipeline's segment-record buffer; 0 = 160 GiB, at
-                               most half the device memory free when the ensemble is seeded */
+                               most half the device memory free when the ensemble is seeded.
+                               With chunk_records 0 the depth (at most 4096 records) comes from
+                               it, plus overflow pages of up to a quarter as many records again
+                               (at most a quarter of the budget) for particles that reach the
+                               depth near the end of the run */
     int32_t chunk_records;  /* trajectory segments per particle per pipeline chunk (and between
-                               adjoint checkpoints); 0 = as many as seg_buffer_mb holds */
+                               adjoint checkpoints); 0 = as many as seg_buffer_mb holds; an
+                               explicit depth gets no overflow pages */
     int32_t graph_chunks;   /* pipeline engine: capture this many chunks (rounded up to even, at
                                most 64) in one HIP graph and let st_advance_to / st_run replay it
                                until every particle is done -- one host round trip per replay
