@@ -388,6 +388,11 @@ __device__ __forceinline__ void spin_load(const DevState& D, const PipeArgs& A, 
     L.t_spin = D.ld(F_TSPIN, o);
     L.S = d3{D.ld(F_SX, o), D.ld(F_SY, o), D.ld(F_SZ, o)};
     L.acc = L.rej = L.seg = 0;
+    // complete these loads here (a particle switch is rare): left in flight,
+    // they put conservative waits into the trial loop, which then also wait
+    // for the other lanes' snapshot and state stores
+    asm volatile("" ::"v"(L.h), "v"(L.tempH), "v"(L.t_spin), "v"(L.S.x), "v"(L.S.y), "v"(L.S.z), "v"(L.nrec),
+                 "v"(ss), "v"(L.cfg));
 }
 
 __device__ __forceinline__ void spin_store(const DevState& D, const SpinLane& L) {
