@@ -87,6 +87,11 @@ struct Phys {
     int32_t pulse_on;     // 0: the pulse term is +0 and may be skipped bit-exactly
     int32_t diffuse_on;
     int32_t dist;         // 'C' or 'M'
+    // The fast variant's segment polynomial (make_poly) in gamma-scaled form,
+    // from the fields above and the gravity switch: gamma*G, gamma*B0,
+    // gamma*E/c^2, the constant part gamma*(a x E)/c^2 of the linear term and
+    // the quadratic term gamma*0.5*G.a, a = (0, -g, 0) or 0; and gamma*pa.
+    double gG[9], gB0[3], gE[3], gw1[3], gw2[3], gpa;
 };
 
 // Run parameters shared by every particle of a launch (geometry, integrator
@@ -674,31 +679,36 @@ ST_HD __forceinline__ d3 poly_rhs(double tau, const d3& y, const Poly& q, double
 
 ST_HD inline Poly make_poly(const SimParams& P, const Phys& F, const Seg& S) {
 #pragma clang fp contract(fast)
-    const double ay = P.gravity ? G_CONST : 0.0;
-    // B0 + G.p0 + (1/c^2) v0 x E
-    const double g0x = S.p0.x * F.G[0] + S.p0.y * F.G[1] + S.p0.z * F.G[2];
-    const double g0y = S.p0.x * F.G[3] + S.p0.y * F.G[4] + S.p0.z * F.G[5];
-    const double g0z = S.p0.x * F.G[6] + S.p0.y * F.G[7] + S.p0.z * F.G[8];
-    const double c0x = S.v0.y * F.Ez - S.v0.z * F.Ey;
-    const double c0y = S.v0.z * F.Ex - S.v0.x * F.Ez;
-    const double c0z = S.v0.x * F.Ey - S.v0.y * F.Ex;
-    // G.v0 + (1/c^2) a x E, a = (0, ay, 0)
-    const double g1x = S.v0.x * F.G[0] + S.v0.y * F.G[1] + S.v0.z * F.G[2];
-    const double g1y = S.v0.x * F.G[3] + S.v0.y * F.G[4] + S.v0.z * F.G[5];
-    const double g1z = S.v0.x * F.G[6] + S.v0.y * F.G[7] + S.v0.z * F.G[8];
-    const double c1x = ay * F.Ez, c1z = -ay * F.Ex;
+    (void)P;
+    // gamma * (B0 + G.p0 + (1/c^2) v0 x E) and gamma * (G.v0 + (1/c^2) a x E),
+    // from the per-handle products of Phys (set_poly_consts)
     Poly q;
-    const double gm = F.gamma;
-    q.w0 = d3{gm * (F.B0x + g0x + c0x * IC2), gm * (F.B0y + g0y + c0y * IC2),
-              gm * (F.B0z + g0z + c0z * IC2)};
-    q.w1 = d3{gm * (g1x + c1x * IC2), gm * g1y, gm * (g1z + c1z * IC2)};
-    // 0.5 * G.a
-    q.w2 = d3{gm * (0.5 * ay * F.G[1]), gm * (0.5 * ay * F.G[4]), gm * (0.5 * ay * F.G[7])};
-    q.pag = gm * F.pa;
+    q.w0 = d3{F.gB0[0] + F.gG[0] * S.p0.x + F.gG[1] * S.p0.y + F.gG[2] * S.p0.z + S.v0.y * F.gE[2] - S.v0.z * F.gE[1],
+              F.gB0[1] + F.gG[3] * S.p0.x + F.gG[4] * S.p0.y + F.gG[5] * S.p0.z + S.v0.z * F.gE[0] - S.v0.x * F.gE[2],
+              F.gB0[2] + F.gG[6] * S.p0.x + F.gG[7] * S.p0.y + F.gG[8] * S.p0.z + S.v0.x * F.gE[1] - S.v0.y * F.gE[0]};
+    q.w1 = d3{F.gw1[0] + F.gG[0] * S.v0.x + F.gG[1] * S.v0.y + F.gG[2] * S.v0.z,
+              F.gw1[1] + F.gG[3] * S.v0.x + F.gG[4] * S.v0.y + F.gG[5] * S.v0.z,
+              F.gw1[2] + F.gG[6] * S.v0.x + F.gG[7] * S.v0.y + F.gG[8] * S.v0.z};
+    q.w2 = d3{F.gw2[0], F.gw2[1], F.gw2[2]};
+    q.pag = F.gpa;
     q.pw = F.pw;
     q.paxis = F.pulse_axis;
     q.pon = F.pulse_on;
     return q;
+}
+
+// The products make_poly reads: filled once per handle (or configuration)
+// wherever a Phys is derived from the options.
+inline void set_poly_consts(Phys& F, bool gravity) {
+    const double gm = F.gamma, ay = gravity ? G_CONST : 0.0;
+    for (int i = 0; i < 9; ++i) F.gG[i] = gm * F.G[i];
+    F.gB0[0] = gm * F.B0x; F.gB0[1] = gm * F.B0y; F.gB0[2] = gm * F.B0z;
+    F.gE[0] = gm * (F.Ex * IC2); F.gE[1] = gm * (F.Ey * IC2); F.gE[2] = gm * (F.Ez * IC2);
+    // (1/c^2) a x E with a = (0, ay, 0)
+    F.gw1[0] = gm * (ay * F.Ez * IC2); F.gw1[1] = 0.0; F.gw1[2] = gm * (-ay * F.Ex * IC2);
+    // 0.5 * G.a
+    F.gw2[0] = gm * (0.5 * ay * F.G[1]); F.gw2[1] = gm * (0.5 * ay * F.G[4]); F.gw2[2] = gm * (0.5 * ay * F.G[7]);
+    F.gpa = gm * F.pa;
 }
 
 ST_HD __forceinline__ d3 rhs_fast(const SimParams& P, const Phys& F, double t, const d3& y, double t0,

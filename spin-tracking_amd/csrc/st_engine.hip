@@ -318,6 +318,7 @@ static Phys derive_phys(const st_options& o, const st_ext& x) {
     F.V = o.V;
     F.yix = o.yi.x; F.yiy = o.yi.y; F.yiz = o.yi.z;
     F.dist = (int32_t)(unsigned char)o.dist;
+    set_poly_consts(F, o.gravity);
     return F;
 }
 

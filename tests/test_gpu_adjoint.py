@@ -70,10 +70,12 @@ def central_fd(f, theta, scale=1.0):
 
 
 def resolution(L, theta, fd):
-    """Relative resolution of a central difference: a few ulps of the loss
-    over the loss change 2*d*|dL/dtheta| (the weakest coupling here moves a
-    ~55 rad loss by ~2e-9 rad)."""
-    return 4.0 * np.spacing(abs(L)) / (2.0 * fd_steps(theta) * np.abs(fd))
+    """Relative resolution of richardson_fd's extrapolated difference.  With
+    each loss evaluation good to e = 2 ulp, a central difference at step d
+    is good to e/d, so (4*g(d/2) - g(d))/3 is good to (4*2e/d + e/d)/3 =
+    3e/d, relative to |dL/dtheta| (the weakest coupling here moves the loss,
+    some tens of rad, by ~2e-9 rad over 2d)."""
+    return 3.0 * 2.0 * np.spacing(abs(L)) / (fd_steps(theta) * np.abs(fd))
 
 
 def richardson_fd(f, theta):
