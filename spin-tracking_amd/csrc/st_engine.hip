@@ -379,14 +379,23 @@ static const void* fused_symbol(int variant, int integ, bool pulse) {
 // The fast DOP853 kernels run at occupancy 3 (the pulse form spilling ~90 B
 // per lane to do so), except the batched pulse form, whose per-lane pulse
 // fields make the spills cost more than the third wave gains; so does the
-// fast MagnusCFET kernel, which sits at the 168-register line.
+// fast MagnusCFET kernel, which sits at the 168-register line.  The fast
+// quaternion kernels of plain handles run at occupancy 4 (128 registers,
+// about 28 B of spills per lane): the fourth wave covers more of the
+// dependent rotation chains than the spills cost.
 template <int VAR, int INTEG, bool BATCH, bool PULSE>
 constexpr bool use_o3() {
     return VAR == VAR_FAST && (INTEG == INT_DOP853 || INTEG == INT_MAGNUS);
 }
 template <int VAR, int INTEG, bool BATCH, bool PULSE>
+constexpr bool use_o4() {
+    return VAR == VAR_FAST && !BATCH && (INTEG == INT_RK45Q || INTEG == INT_RKF45Q);
+}
+template <int VAR, int INTEG, bool BATCH, bool PULSE>
 static hipError_t launch_spin(st_handle* h, dim3 grid, const PipeArgs& A) {
-    if constexpr (use_o3<VAR, INTEG, BATCH, PULSE>())
+    if constexpr (use_o4<VAR, INTEG, BATCH, PULSE>())
+        hipLaunchKernelGGL((k_spin_o4<VAR, INTEG, BATCH, PULSE>), grid, dim3(256), 0, h->stream, h->P, h->D, A);
+    else if constexpr (use_o3<VAR, INTEG, BATCH, PULSE>())
         hipLaunchKernelGGL((k_spin_o3<VAR, INTEG, BATCH, PULSE>), grid, dim3(256), 0, h->stream, h->P, h->D, A);
     else
         hipLaunchKernelGGL((k_spin<VAR, INTEG, BATCH, PULSE>), grid, dim3(256), 0, h->stream, h->P, h->D, A);
@@ -413,24 +422,28 @@ static SpinFn pick_spin(int variant, int integ, bool batch, bool pulse) {
         return batch ? pick_spin_v<VAR_FAST, true>(integ, pulse) : pick_spin_v<VAR_FAST, false>(integ, pulse);
     return batch ? pick_spin_v<VAR_EXACT, true>(integ, pulse) : pick_spin_v<VAR_EXACT, false>(integ, pulse);
 }
+template <int VAR, int INTEG, bool BATCH, bool PULSE>
+static const void* spin_sym() {
+    if constexpr (use_o4<VAR, INTEG, BATCH, PULSE>())
+        return (const void*)&k_spin_o4<VAR, INTEG, BATCH, PULSE>;
+    else if constexpr (use_o3<VAR, INTEG, BATCH, PULSE>())
+        return (const void*)&k_spin_o3<VAR, INTEG, BATCH, PULSE>;
+    else
+        return (const void*)&k_spin<VAR, INTEG, BATCH, PULSE>;
+}
 template <int VAR, bool BATCH>
 static const void* spin_symbol_v(int integ, bool pulse) {
     switch (integ) {
-    case INT_RK45: return (const void*)&k_spin<VAR, INT_RK45, BATCH, true>;
-    case INT_MAGNUS:
-        return use_o3<VAR, INT_MAGNUS, BATCH, true>() ? (const void*)&k_spin_o3<VAR, INT_MAGNUS, BATCH, true>
-                                                      : (const void*)&k_spin<VAR, INT_MAGNUS, BATCH, true>;
-    case INT_RK45Q: return (const void*)&k_spin<VAR, INT_RK45Q, BATCH, true>;
-    case INT_RKF45: return (const void*)&k_spin<VAR, INT_RKF45, BATCH, true>;
-    case INT_RKF45Q: return (const void*)&k_spin<VAR, INT_RKF45Q, BATCH, true>;
+    case INT_RK45: return spin_sym<VAR, INT_RK45, BATCH, true>();
+    case INT_MAGNUS: return spin_sym<VAR, INT_MAGNUS, BATCH, true>();
+    case INT_RK45Q: return spin_sym<VAR, INT_RK45Q, BATCH, true>();
+    case INT_RKF45: return spin_sym<VAR, INT_RKF45, BATCH, true>();
+    case INT_RKF45Q: return spin_sym<VAR, INT_RKF45Q, BATCH, true>();
     default:
         if constexpr (VAR == VAR_FAST)
-            return pulse ? (use_o3<VAR, INT_DOP853, BATCH, true>()
-                                ? (const void*)&k_spin_o3<VAR, INT_DOP853, BATCH, true>
-                                : (const void*)&k_spin<VAR, INT_DOP853, BATCH, true>)
-                         : (const void*)&k_spin_o3<VAR, INT_DOP853, BATCH, false>;
+            return pulse ? spin_sym<VAR, INT_DOP853, BATCH, true>() : spin_sym<VAR, INT_DOP853, BATCH, false>();
         else
-            return (const void*)&k_spin<VAR, INT_DOP853, BATCH, true>;
+            return spin_sym<VAR, INT_DOP853, BATCH, true>();
     }
 }
 static const void* spin_symbol(int variant, int integ, bool batch, bool pulse) {

@@ -568,6 +568,11 @@ template <int VAR, int INTEG, bool BATCH, bool PULSE>
 __global__ __launch_bounds__(256, 3) void k_spin_o3(const SimParams P, const DevState D, const PipeArgs A) {
     spin_body<VAR, INTEG, BATCH, PULSE>(P, D, A);
 }
+// ... and held to 128 (four waves per SIMD), for the quaternion trials
+template <int VAR, int INTEG, bool BATCH, bool PULSE>
+__global__ __launch_bounds__(256, 4) void k_spin_o4(const SimParams P, const DevState D, const PipeArgs A) {
+    spin_body<VAR, INTEG, BATCH, PULSE>(P, D, A);
+}
 
 // Rare path: a spin-integration failure at record j_fail of this chunk.
 // Rewind the kinematics to the chunk start, replay exactly j_fail+1 records
