@@ -691,23 +691,34 @@ ST_HD __forceinline__ d3 rodriguez(const d3& k, const d3& v) {
 // and non-finite vectors take the exact form.  The values agree with the
 // exact form to a few units in the last place per doubling, well inside the
 // integrator tolerance the fast variant is held to.
+// (Q: for x^2 <= 1/4 only, 8 and 8 terms, first omitted terms below 1e-18)
+template <bool Q = false>
 ST_HD __forceinline__ void half_series(double x2, double& c, double& s) {
-    c = taylor_sum<0, 11>(x2);
-    s = taylor_sum<1, 10>(x2);
+    if constexpr (Q) {
+        c = taylor_sum<0, 8>(x2);
+        s = taylor_sum<1, 8>(x2);
+    } else {
+        c = taylor_sum<0, 11>(x2);
+        s = taylor_sum<1, 10>(x2);
+    }
 }
 constexpr double FAST_ROT_MAX_X2 = 0x1p60;   // at most 31 doublings
 // cos x and sin(x)/x for x^2 = X2 in (0, 2^60]: the series after d >= D0
-// halvings, d the fewest that bring the square to 1 or below
+// halvings, d the fewest that bring the square to 1 (Q: to 1/4) or below
 // (VOTE: the wave decides once, with a scalar branch, whether any of its
 // lanes needs more than D0 halvings, so the common case pays one multiply
 // for the scaling; the branch waits for the compare, which costs more than
-// it saves in the MagnusCFET chain and less in the quaternion trials.)
-template <int D0, bool VOTE>
+// it saves in the MagnusCFET chain and less in the quaternion trials.
+// Q trades five series terms for one more halving and doubling in the lanes
+// whose square exceeds 1/4 after D0 halvings: the MagnusCFET exponents are
+// mostly below it.)
+template <int D0, bool VOTE, bool Q = false>
 ST_HD __forceinline__ void cos_sinc_doubled(double X2, double& c, double& s) {
 #pragma clang fp contract(fast)
     constexpr double SCALE = D0 == 0 ? 1.0 : D0 == 1 ? 0.25 : 0.0625;   // 4^-D0
+    constexpr int QE = Q ? 2 : 0;                                         // the range is 4^-QE/2
     static_assert(D0 >= 0 && D0 <= 2, "D0");
-    const bool extra = X2 > 1.0 / SCALE;
+    const bool extra = X2 > (Q ? 0.25 : 1.0) / SCALE;
 #if defined(__HIP_DEVICE_COMPILE__)
     const bool wave_extra = VOTE ? __any(extra) : true;
 #else
@@ -717,11 +728,11 @@ ST_HD __forceinline__ void cos_sinc_doubled(double X2, double& c, double& s) {
     double x2 = X2 * SCALE;
     if (wave_extra) {
         int E;
-        (void)frexp(X2, &E);   // X2 < 2^E, so X2 / 4^d < 1 once 2d >= E
-        d = E > 2 * D0 ? (E + 1) >> 1 : D0;
+        (void)frexp(X2, &E);   // X2 < 2^E, so X2 / 4^d < 2^-QE once 2d >= E + QE
+        d = E + QE > 2 * D0 ? (E + QE + 1) >> 1 : D0;
         x2 = ldexp(X2, -2 * d);
     }
-    half_series(x2, c, s);
+    half_series<Q>(x2, c, s);
     auto twice = [&]() {
         const double s2 = s * s;
         s = s * c;
@@ -753,26 +764,27 @@ ST_HD __forceinline__ d3 rodriguez_v(const d3& k, const d3& v) {
     }
     return rodriguez(k, v);
 }
-// Fast quaternion forms for chains of rotations.  rot_quat_fast(k) is the
-// unit quaternion of rodriguez(k, .) -- a rotation by theta = |k| about
-// -k/|k|: (cos(theta/2), -(k/2) sin(theta/2)/(theta/2)) -- from the same
+// Fast quaternion forms for chains of rotations.  rot_quat_half(g), g = -k/2,
+// is the unit quaternion of rodriguez(k, .) -- a rotation by theta = |k|
+// about -k/|k|: (cos(theta/2), g sin(theta/2)/(theta/2)) -- from the same
 // series and doublings as rodriguez_v (one doubling fewer: the half angle is
-// what a quaternion holds); the vectors the exact form takes go through
-// rodriguez_quat(k, 1), which keeps the reference's NaN for a zero or
-// non-finite k.  A chain of rotations is then composed with 16-operation
-// quaternion products and applied to the vector once (q v q* as
-// v + w t + u x t, t = 2 u x v), in place of a 30-operation vector rotation
-// per factor; the same rotations, rounded differently.
-ST_HD __forceinline__ Quat rot_quat_fast(const d3& k) {
+// what a quaternion holds).  The caller forms g instead of k (the factor -1/2
+// is exact, so g.g is exactly (theta/2)^2 and g*s exactly -(k/2)*s); the
+// vectors the exact form takes go through rodriguez_quat(k, 1), which keeps
+// the reference's NaN for a zero or non-finite k.  A chain of rotations is
+// then composed with 16-operation quaternion products and applied to the
+// vector once (q v q* as v + w t + u x t, t = 2 u x v), in place of a
+// 30-operation vector rotation per factor; the same rotations, rounded
+// differently.
+ST_HD __forceinline__ Quat rot_quat_half(const d3& g) {
 #pragma clang fp contract(fast)
-    const double t2 = k.x * k.x + k.y * k.y + k.z * k.z;
-    if (t2 > 0.0 && t2 <= FAST_ROT_MAX_X2) {
+    const double t2 = g.x * g.x + g.y * g.y + g.z * g.z;   // (theta/2)^2
+    if (t2 > 0.0 && t2 <= 0.25 * FAST_ROT_MAX_X2) {
         double ch, sh;
-        cos_sinc_doubled<1, false>(0.25 * t2, ch, sh);
-        const double s = -0.5 * sh;
-        return Quat{ch, k.x * s, k.y * s, k.z * s};
+        cos_sinc_doubled<1, false, true>(t2, ch, sh);
+        return Quat{ch, g.x * sh, g.y * sh, g.z * sh};
     }
-    return rodriguez_quat(k, 1.0);
+    return rodriguez_quat(d3{-2.0 * g.x, -2.0 * g.y, -2.0 * g.z}, 1.0);
 }
 ST_HD __forceinline__ Quat q_mult_fast(const Quat& a, const Quat& b) {
 #pragma clang fp contract(fast)
@@ -843,9 +855,16 @@ ST_HD __forceinline__ d3 cfet_exp(double c1, double c2, double c3, double c4, do
 // order-8 chain k11 .. k1 (:688-700), then the order-6 chain k5 .. k1
 // (:702-708).  Read with a uniform index by a rolled loop: one copy of the
 // rotation code instead of sixteen keeps the kernel's registers in check.
-__constant__ double CFET_ROWS[16][5] = {{ST_G1R}, {ST_G2R}, {ST_G3R}, {ST_G4R}, {ST_G5R}, {ST_G6},
-                                        {ST_G5},  {ST_G4},  {ST_G3},  {ST_G2},  {ST_G1}, {ST_H1R},
-                                        {ST_H2R}, {ST_H3},  {ST_H2},  {ST_H1}};
+// (Constant memory in the device pass; a plain table in the host pass, where
+// a constant-memory variable would need the device image registered.)
+#define ST_CFET_ROWS                                                                                   \
+    {{ST_G1R}, {ST_G2R}, {ST_G3R}, {ST_G4R}, {ST_G5R}, {ST_G6}, {ST_G5}, {ST_G4}, {ST_G3}, {ST_G2}, {ST_G1}, \
+     {ST_H1R}, {ST_H2R}, {ST_H3},  {ST_H2},  {ST_H1}}
+#if defined(__HIP_DEVICE_COMPILE__)
+__constant__ double CFET_ROWS[16][5] = ST_CFET_ROWS;
+#else
+constexpr double CFET_ROWS[16][5] = ST_CFET_ROWS;
+#endif
 
 // The same rows as moments for the fast variant.  Along a segment gamma*B is
 // the quadratic w(tau) = w0 + w1 tau + w2 tau^2, so an exponent
@@ -857,10 +876,9 @@ __constant__ double CFET_ROWS[16][5] = {{ST_G1R}, {ST_G2R}, {ST_G3R}, {ST_G4R}, 
 struct CfetMoments {
     double m[16][8];
 };
-constexpr CfetMoments cfet_moments() {
-    constexpr double rows[16][5] = {{ST_G1R}, {ST_G2R}, {ST_G3R}, {ST_G4R}, {ST_G5R}, {ST_G6},
-                                    {ST_G5},  {ST_G4},  {ST_G3},  {ST_G2},  {ST_G1}, {ST_H1R},
-                                    {ST_H2R}, {ST_H3},  {ST_H2},  {ST_H1}};
+// (every entry times `scale`: -1/2 gives rot_quat_half's argument directly)
+constexpr CfetMoments cfet_moments(double scale) {
+    constexpr double rows[16][5] = ST_CFET_ROWS;
     constexpr double X[5] = {cfet::X1, cfet::X2, cfet::X3, cfet::X4, cfet::X5};
     CfetMoments M{};
     for (int r = 0; r < 16; ++r) {
@@ -870,13 +888,14 @@ constexpr CfetMoments cfet_moments() {
             M.m[r][2] += rows[r][i] * X[i] * X[i];
             M.m[r][3 + i] = rows[r][i];
         }
+        for (int j = 0; j < 8; ++j) M.m[r][j] *= scale;
     }
     return M;
 }
 #if defined(__HIP_DEVICE_COMPILE__)
-__constant__ CfetMoments CFET_MOM = cfet_moments();
+__constant__ CfetMoments CFET_MOM = cfet_moments(-0.5);
 #endif
-constexpr CfetMoments CFET_MOM_HOST = cfet_moments();
+constexpr CfetMoments CFET_MOM_HOST = cfet_moments(-0.5);
 
 // Quaternion integrators, per stage i = 0..5: node A_i, weight W_i of the
 // step, weight E_i of the comparison solution (E_5 unused); [0] RK45
@@ -887,7 +906,11 @@ constexpr CfetMoments CFET_MOM_HOST = cfet_moments();
      {{0.0, 16.0 / 135.0, 16.0 / 135.0}, {1.0 / 4.0, 0.0, 0.0}, {3.0 / 8.0, 6656.0 / 12825.0, 6656.0 / 12825.0},   \
       {12.0 / 13.0, 28561.0 / 56430.0, 28561.0 / 56430.0}, {1.0, -9.0 / 50.0, -9.0 / 50.0},                      \
       {0.5, 2.0 / 55.0, 0.0}}}
+#if defined(__HIP_DEVICE_COMPILE__)
 __constant__ double QUAT_TAB[2][6][3] = ST_QUAT_TAB;
+#else
+constexpr double QUAT_TAB[2][6][3] = ST_QUAT_TAB;
+#endif
 // the same table for compile-time decisions of the fast variant
 struct QuatTab {
     double q[2][6][3];
@@ -1096,7 +1119,8 @@ ST_HD inline int rk2_trial(const SimParams& P, const Phys& Fp, RkState& R, d3& y
         using namespace cfet;
         if constexpr (VAR == VAR_FAST) {
 #pragma clang fp contract(fast)
-            // the exponents from the segment polynomial's moments (CFET_MOM)
+            // the exponents from the segment polynomial's moments (CFET_MOM,
+            // scaled by -1/2 for rot_quat_half)
             const double tau = t - S.t0, h2 = h * h, h3 = h2 * h, tt = 2.0 * tau;
             const d3 u0{h * (q.w0.x + tau * (q.w1.x + tau * q.w2.x)), h * (q.w0.y + tau * (q.w1.y + tau * q.w2.y)),
                         h * (q.w0.z + tau * (q.w1.z + tau * q.w2.z))};
@@ -1112,7 +1136,12 @@ ST_HD inline int rk2_trial(const SimParams& P, const Phys& Fp, RkState& R, d3& y
             }
             auto expo = [&](int r) -> d3 {
 #if defined(__HIP_DEVICE_COMPILE__)
-                const double* m = CFET_MOM.m[r];
+                // an opaque constant-space pointer: the row's scalar loads stay
+                // here (the two rows of the first factors would otherwise be
+                // hoisted out of the lane loop and spilled to vector lanes)
+                typedef const __attribute__((address_space(4))) double cdouble;
+                cdouble* m = (cdouble*)CFET_MOM.m[r];
+                asm volatile("" : "+s"(m));
 #else
                 const double* m = CFET_MOM_HOST.m[r];
 #endif
@@ -1126,12 +1155,14 @@ ST_HD inline int rk2_trial(const SimParams& P, const Phys& Fp, RkState& R, d3& y
                 }
                 return k;
             };
-            // both chains as quaternion products (the first factor acts first)
-            Quat q8{1.0, 0.0, 0.0, 0.0}, q6{1.0, 0.0, 0.0, 0.0};
+            // both chains as quaternion products (the first factor acts
+            // first; each chain starts at its first factor, which equals that
+            // factor times the identity exactly)
+            Quat q8 = rot_quat_half(expo(0)), q6 = rot_quat_half(expo(11));
 #pragma unroll 1
-            for (int r = 0; r < 11; ++r) {
-                q8 = q_mult_fast(rot_quat_fast(expo(r)), q8);
-                if (r < 5) q6 = q_mult_fast(rot_quat_fast(expo(11 + r)), q6);
+            for (int r = 1; r < 11; ++r) {
+                q8 = q_mult_fast(rot_quat_half(expo(r)), q8);
+                if (r < 5) q6 = q_mult_fast(rot_quat_half(expo(11 + r)), q6);
             }
             const d3 y8 = qrot_fast(q8, y), y6 = qrot_fast(q6, y);
             const d3 d{y8.x - y6.x, y8.y - y6.y, y8.z - y6.z};

@@ -12,7 +12,8 @@
 //     interval cap, +-0, +-inf, NaN and tiny velocity components, positions on
 //     the walls, an apex exactly at the ceiling.
 // The fast variant's step-size log and exp (ctl_log / ctl_exp) are checked
-// against the C library to 1e-15 relative.
+// against the C library to 1e-15 relative, and its rotation series
+// (cos_sinc_doubled) against long double cos and sin to 2e-15 absolute.
 // Prints "cases N mismatches M" and exits non-zero on any mismatch.
 //
 // Diffuse re-emission takes sin and cos of two random angles: the device code
@@ -29,6 +30,7 @@
 #include <random>
 
 #include "../spin-tracking_amd/csrc/st_device.h"
+#include "../spin-tracking_amd/csrc/st_integrators.h"
 #include "../oracle/st_oracle.h"
 
 static_assert(sizeof(st::Rng) == sizeof(or_rng), "rng layout");
@@ -225,6 +227,32 @@ int main(int argc, char** argv) {
         check(std::isnan(st::ctl_exp(NAN)) && st::ctl_exp(-1e6) == 0.0 && std::isinf(st::ctl_exp(1e6)),
               "ctl_exp special", 0);
         printf("ctl_log worst relative error %.3g, ctl_exp %.3g\n", worst_log, worst_exp);
+    }
+    // ---- the fast rotations' cos x and sin(x)/x (st_integrators.h
+    // cos_sinc_doubled) in the three forms the trials use, against long
+    // double cos and sin: x up to 4 (at most two extra doublings), absolute
+    // error of cos x and of sin(x)/x times max(x, 1) (the rotation's
+    // components) within 2e-15 ----
+    {
+        std::mt19937_64 g3(9);
+        std::uniform_real_distribution<double> ux(-30.0, 2.0);   // log2 x
+        double worst = 0.0;
+        for (long i = 0; i < n; ++i) {
+            const double x = std::exp2(ux(g3)), X2 = x * x;
+            const long double cr = std::cos((long double)x), sr = std::sin((long double)x) / (long double)x;
+            double c[3], sv[3];
+            st::cos_sinc_doubled<1, false, false>(X2, c[0], sv[0]);
+            st::cos_sinc_doubled<1, false, true>(X2, c[1], sv[1]);
+            st::cos_sinc_doubled<0, true, false>(X2, c[2], sv[2]);
+            for (int f = 0; f < 3; ++f) {
+                const double ec = (double)std::fabs((long double)c[f] - cr);
+                const double es = (double)std::fabs((long double)sv[f] - sr) * std::fmax(x, 1.0);
+                worst = std::fmax(worst, std::fmax(ec, es));
+                check(ec <= 2e-15 && es <= 2e-15, "cos_sinc_doubled", i);
+            }
+            ++cases;
+        }
+        printf("rotation series worst error %.3g\n", worst);
     }
     printf("diffuse events %ld (last-bit differences %ld)\n", g_diffuse, g_diffuse_inexact);
     printf("gas jobs %ld\n", g_gas_jobs);
