@@ -710,8 +710,8 @@ constexpr double FAST_ROT_MAX_X2 = 0x1p60;   // at most 31 doublings
 // for the scaling; the branch waits for the compare, which costs more than
 // it saves in the MagnusCFET chain and less in the quaternion trials.
 // Q trades five series terms for one more halving and doubling in the lanes
-// whose square exceeds 1/4 after D0 halvings: the MagnusCFET exponents are
-// mostly below it.)
+// whose square exceeds 1/4 after D0 halvings: the MagnusCFET exponents and
+// the quaternion stages' half angles are mostly below it.)
 template <int D0, bool VOTE, bool Q = false>
 ST_HD __forceinline__ void cos_sinc_doubled(double X2, double& c, double& s) {
 #pragma clang fp contract(fast)
@@ -809,7 +809,7 @@ ST_HD __forceinline__ Quat rodriguez_quat_v(const d3& k, double dt) {
         const double p2 = (k.x * k.x + k.y * k.y + k.z * k.z) * (hd * hd);
         if (p2 > 0.0 && p2 <= FAST_ROT_MAX_X2) {
             double c, sp;
-            cos_sinc_doubled<0, true>(p2, c, sp);
+            cos_sinc_doubled<0, true, true>(p2, c, sp);
             const double s = -(hd * sp);
             return Quat{c, k.x * s, k.y * s, k.z * s};
         }
