@@ -54,12 +54,12 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE / WRITE_SIZE counters of the pipeline on this workload, scaled by
 # a 1 GiB device copy (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000); the
 # algorithmic figure is 64 B per record plus the per-particle spin state.
-SPIN_HBM_BYTES_PER_RECORD = 74.6
+SPIN_HBM_BYTES_PER_RECORD = 74.5
 # The same kernel's executed FP64 rate from the hardware counters
 # (SQ_INSTS_VALU_{FMA,MUL,ADD,TRANS}_F64, an FMA counted as 2 operations) over
 # its rocprofv3 duration on this workload; the fast variant executes fewer
 # FP64 operations per step than the reference formulation above counts.
-HW_COUNTED = {"frac_fp64_peak": 0.468, "TFLOPs": 36.8,
+HW_COUNTED = {"frac_fp64_peak": 0.471, "TFLOPs": 37.0,
               "source": "profiles/k_spin_o3_k_kin_fast_dop853_he3_1M_2intervals.md"}
 
 
