@@ -1158,11 +1158,19 @@ ST_HD inline int rk2_trial(const SimParams& P, const Phys& Fp, RkState& R, d3& y
             // both chains as quaternion products (the first factor acts
             // first; each chain starts at its first factor, which equals that
             // factor times the identity exactly)
+            // (two independent rotations per loop trip: the two chains side
+            // by side while the order-6 chain lasts, then two consecutive
+            // order-8 factors, whose products with the chain follow in order)
             Quat q8 = rot_quat_half(expo(0)), q6 = rot_quat_half(expo(11));
 #pragma unroll 1
-            for (int r = 1; r < 11; ++r) {
+            for (int r = 1; r < 5; ++r) {
                 q8 = q_mult_fast(rot_quat_half(expo(r)), q8);
-                if (r < 5) q6 = q_mult_fast(rot_quat_half(expo(11 + r)), q6);
+                q6 = q_mult_fast(rot_quat_half(expo(11 + r)), q6);
+            }
+#pragma unroll 1
+            for (int r = 5; r < 11; r += 2) {
+                const Quat a = rot_quat_half(expo(r)), b = rot_quat_half(expo(r + 1));
+                q8 = q_mult_fast(b, q_mult_fast(a, q8));
             }
             const d3 y8 = qrot_fast(q8, y), y6 = qrot_fast(q6, y);
             const d3 d{y8.x - y6.x, y8.y - y6.y, y8.z - y6.z};
