@@ -995,8 +995,7 @@ ST_HD __forceinline__ int control_v2(const SimParams& P, RkState& R, d3& y, cons
 // second stage of both tableaus) contributes the reference's
 // rodriguez_quat(k, 0): the identity for a finite nonzero k (multiplying by
 // it changes no nonzero component), NaN for a zero or non-finite k (its
-// 0/0); the factor is not formed, only that test is kept (the weights are
-// read from constant memory, so these branches are uniform).  Where
+// 0/0); the factor is not formed, only that test is kept.  Where
 // W_i = E_i for the first five stages (RKF45) the two chains share their
 // factors: the comparison product is the step's product after stage five.
 template <int INTEG, typename WF>
@@ -1010,23 +1009,40 @@ ST_HD __forceinline__ int quat_trial_fast(const SimParams& P, RkState& R, d3& y,
                             QUAT_TAB_C.q[TI][3][1] == QUAT_TAB_C.q[TI][3][2] &&
                             QUAT_TAB_C.q[TI][4][1] == QUAT_TAB_C.q[TI][4][2];
     static_assert(QUAT_TAB_C.q[TI][0][1] != 0.0 && QUAT_TAB_C.q[TI][0][2] != 0.0, "first factors start the chains");
+    // the second stage carries weight 0 in both chains, every later stage
+    // a nonzero one (the comparison chain ends after stage five)
+    static_assert(QUAT_TAB_C.q[TI][1][1] == 0.0 && QUAT_TAB_C.q[TI][1][2] == 0.0, "stage 1 has weight 0");
+    static_assert(QUAT_TAB_C.q[TI][2][1] != 0.0 && QUAT_TAB_C.q[TI][3][1] != 0.0 && QUAT_TAB_C.q[TI][4][1] != 0.0 &&
+                      QUAT_TAB_C.q[TI][5][1] != 0.0 && QUAT_TAB_C.q[TI][2][2] != 0.0 &&
+                      QUAT_TAB_C.q[TI][3][2] != 0.0 && QUAT_TAB_C.q[TI][4][2] != 0.0,
+                  "stages 2-5 have nonzero weights");
     const double(*Q)[3] = QUAT_TAB[TI];
     d3 k = W(t);
     Quat qw = rodriguez_quat_v<VAR_FAST>(k, h * Q[0][1]);
     Quat qe = SHARED ? qw : rodriguez_quat_v<VAR_FAST>(k, h * Q[0][2]);
-    bool degenerate = false;   // a zero-weight factor met a zero or non-finite k
+    bool degenerate = false;   // stage 1's zero-weight factors met a zero or non-finite k
+    // (two forms of the same loop: the weight tests read from the table in
+    // the two-chain trial, a branch on the stage index in the shared-chain
+    // one; each is the form its kernel's register allocation takes best)
 #pragma unroll 1
     for (int i = 1; i < 6; ++i) {
         k = W(t + Q[i][0] * h);
-        const double ww = Q[i][1], we = i < 5 ? Q[i][2] : 1.0;
-        if (ww == 0.0 || we == 0.0) {
-            const double t2 = k.x * k.x + k.y * k.y + k.z * k.z;
-            degenerate = degenerate || !(t2 > 0.0 && t2 <= DBL_MAX_D);
-        }
-        if (ww != 0.0) qw = q_mult_fast(rodriguez_quat_v<VAR_FAST>(k, h * ww), qw);
-        if (i < 5) {
-            if constexpr (SHARED) qe = qw;
-            else if (we != 0.0) qe = q_mult_fast(rodriguez_quat_v<VAR_FAST>(k, h * we), qe);
+        if constexpr (SHARED) {
+            if (i == 1) {   // only the test of its factors remains
+                const double t2 = k.x * k.x + k.y * k.y + k.z * k.z;
+                degenerate = !(t2 > 0.0 && t2 <= DBL_MAX_D);
+                continue;
+            }
+            qw = q_mult_fast(rodriguez_quat_v<VAR_FAST>(k, h * Q[i][1]), qw);
+            if (i < 5) qe = qw;
+        } else {
+            const double ww = Q[i][1], we = i < 5 ? Q[i][2] : 1.0;
+            if (ww == 0.0 || we == 0.0) {
+                const double t2 = k.x * k.x + k.y * k.y + k.z * k.z;
+                degenerate = degenerate || !(t2 > 0.0 && t2 <= DBL_MAX_D);
+            }
+            if (ww != 0.0) qw = q_mult_fast(rodriguez_quat_v<VAR_FAST>(k, h * ww), qw);
+            if (i < 5 && we != 0.0) qe = q_mult_fast(rodriguez_quat_v<VAR_FAST>(k, h * we), qe);
         }
     }
     if (degenerate) {
