@@ -681,16 +681,19 @@ ST_HD __forceinline__ d3 rodriguez(const d3& k, const d3& v) {
 // sum), taken at x / 2^d and doubled d times: for an angle 2y,
 // sin(2y)/(2y) = s c, (1 - cos 2y)/(2y)^2 = s^2/2 (no cancellation) and
 // cos 2y = 1 - (2y)^2 s^2/2.  A Rodrigues rotation by theta = |k| takes them
-// at theta/4 and doubles twice (within 6 units in the last place of libm for
-// theta <= 4); a quaternion takes them at its own half angle.  Longer steps
+// at theta/4 and doubles twice; a quaternion takes them at its own half
+// angle.  Longer steps
 // (tens of radians in one exponent: a smooth field lets the controller grow
 // the step to the segment's end) halve further and take the extra doublings
 // in a short loop of their own, a few ulp per doubling -- in place of the
 // library's argument reduction, which one such lane would otherwise impose
 // on its whole wave.  Zero vectors (the reference's 0/0), squares above 2^60
-// and non-finite vectors take the exact form.  The values agree with the
-// exact form to a few units in the last place per doubling, well inside the
-// integrator tolerance the fast variant is held to.
+// and non-finite vectors take the exact form.  Absolute errors against the
+// exact form (the C library's sin and cos): cos x and sin(x)/x within 2e-15
+// for x <= 4; quaternion components within 1.5e-15 for half angles up to 2,
+// 8e-15 up to 8, 2.5e-14 up to 16 (each extra doubling adds a few ulp), well
+// inside the integrator tolerance the fast variant is held to
+// (tests/host_device_math_main.cpp checks these bounds).
 // (Q: for x^2 <= 1/4 only, 8 and 8 terms, first omitted terms below 1e-18)
 template <bool Q = false>
 ST_HD __forceinline__ void half_series(double x2, double& c, double& s) {

@@ -12,8 +12,9 @@
 //     interval cap, +-0, +-inf, NaN and tiny velocity components, positions on
 //     the walls, an apex exactly at the ceiling.
 // The fast variant's step-size log and exp (ctl_log / ctl_exp) are checked
-// against the C library to 1e-15 relative, and its rotation series
-// (cos_sinc_doubled) against long double cos and sin to 2e-15 absolute.
+// against the C library to 1e-15 relative, its rotation series
+// (cos_sinc_doubled) against long double cos and sin to 2e-15 absolute, and
+// its rotation quaternions against the exact form (bounds below).
 // Prints "cases N mismatches M" and exits non-zero on any mismatch.
 //
 // Diffuse re-emission takes sin and cos of two random angles: the device code
@@ -253,6 +254,51 @@ int main(int argc, char** argv) {
             ++cases;
         }
         printf("rotation series worst error %.3g\n", worst);
+    }
+    // ---- the fast rotation quaternions (rodriguez_quat_v<fast>, used by the
+    // quaternion trials, and rot_quat_half, used by MagnusCFET) against the
+    // exact form rodriguez_quat (the reference's, with the C library's sin and
+    // cos): half angles p from 3e-8 to 16, absolute error of each component
+    // within 1.5e-15 for p <= 2, 8e-15 for p <= 8 and 2.5e-14 above (the extra
+    // doublings add a few ulp each); a zero or non-finite vector gives the
+    // exact form's values (NaN components) ----
+    {
+        std::mt19937_64 g4(11);
+        std::uniform_real_distribution<double> uu(-1.0, 1.0), lp(-25.0, 4.0);
+        double worst[3] = {0.0, 0.0, 0.0};
+        const double bound[3] = {1.5e-15, 8e-15, 2.5e-14};
+        auto diff = [](const st::Quat& a, const st::Quat& b) {
+            return std::fmax(std::fmax(std::fabs(a.w - b.w), std::fabs(a.x - b.x)),
+                             std::fmax(std::fabs(a.y - b.y), std::fabs(a.z - b.z)));
+        };
+        for (long i = 0; i < n; ++i) {
+            const st::d3 d{uu(g4), uu(g4), uu(g4)};
+            const double dn = std::sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+            if (!(dn > 1e-3)) continue;
+            const double p = std::exp2(lp(g4)), dt = (i & 1) ? 1.0 : 0.37, th = 2.0 * p / dt;
+            const st::d3 k{d.x / dn * th, d.y / dn * th, d.z / dn * th};
+            const st::Quat qe = st::rodriguez_quat(k, dt);
+            const double e1 = diff(st::rodriguez_quat_v<st::VAR_FAST>(k, dt), qe);
+            const double e2 = diff(st::rot_quat_half(st::d3{-0.5 * dt * k.x, -0.5 * dt * k.y, -0.5 * dt * k.z}),
+                                   st::rodriguez_quat(st::d3{dt * k.x, dt * k.y, dt * k.z}, 1.0));
+            const int b = p <= 2.0 ? 0 : (p <= 8.0 ? 1 : 2);
+            worst[b] = std::fmax(worst[b], std::fmax(e1, e2));
+            check(e1 <= bound[b] && e2 <= bound[b], "fast rotation quaternion", i);
+            ++cases;
+        }
+        auto same_q = [](const st::Quat& a, const st::Quat& b) {
+            const double x[4] = {a.w, a.x, a.y, a.z}, y[4] = {b.w, b.x, b.y, b.z};
+            for (int c = 0; c < 4; ++c)
+                if (std::isnan(x[c]) != std::isnan(y[c]) || (!std::isnan(x[c]) && x[c] != y[c])) return false;
+            return true;
+        };
+        const st::d3 odd[3] = {{0.0, 0.0, 0.0}, {NAN, 0.0, 0.0}, {0.0, INFINITY, 0.0}};
+        for (const st::d3& k : odd) {
+            check(same_q(st::rodriguez_quat_v<st::VAR_FAST>(k, 1.0), st::rodriguez_quat(k, 1.0)), "quat_v special", 0);
+            check(same_q(st::rot_quat_half(st::d3{-0.5 * k.x, -0.5 * k.y, -0.5 * k.z}), st::rodriguez_quat(k, 1.0)),
+                  "rot_quat_half special", 0);
+        }
+        printf("rotation quaternion worst error %.3g / %.3g / %.3g\n", worst[0], worst[1], worst[2]);
     }
     printf("diffuse events %ld (last-bit differences %ld)\n", g_diffuse, g_diffuse_inexact);
     printf("gas jobs %ld\n", g_gas_jobs);
