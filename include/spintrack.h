@@ -91,10 +91,12 @@ typedef struct st_ext {
                                With chunk_records 0 the depth (at most 4096 records) comes from
                                it, plus overflow pages of up to a quarter as many records again
                                (at most a quarter of the budget) for particles that reach the
-                               depth near the end of the run */
+                               depth near the end of the run.  Trajectory-only handles
+                               (integratorType -1) keep no records and ignore it */
     int32_t chunk_records;  /* trajectory segments per particle per pipeline chunk (and between
                                adjoint checkpoints); 0 = as many as seg_buffer_mb holds; an
-                               explicit depth gets no overflow pages */
+                               explicit depth gets no overflow pages; ignored by
+                               trajectory-only handles, which run each call in one chunk */
     int32_t graph_chunks;   /* pipeline engine: capture this many chunks (rounded up to even, at
                                most 64) in one HIP graph and let st_advance_to / st_run replay it
                                until every particle is done -- one host round trip per replay

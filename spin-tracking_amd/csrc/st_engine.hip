@@ -218,6 +218,7 @@ struct st_handle {
     // pipeline buffers
     char* d_recs = nullptr;
     uint32_t rec_cap = 0;          // C
+    bool no_recs = false;          // trajectory-only: k_kin writes no records
     char* d_ovf = nullptr;         // overflow pages (plain pipeline runs)
     uint32_t* d_ovf_first = nullptr;
     uint32_t* d_ovf_link = nullptr;
@@ -510,6 +511,14 @@ static int ensure_pipeline(st_handle* h) {
     if (h->ext.chunk_records > 0) C = (uint64_t)h->ext.chunk_records;
     C = std::min<uint64_t>(std::max<uint64_t>(C, 16), 1u << 20);
     if (h->ext.chunk_records == 0) pool = std::min<uint64_t>(C * n / 4, budget / 4 / sizeof(SegRec));
+    // Trajectory-only runs read no records: none are kept (a token buffer
+    // stands for them), so such a handle holds no record budget and the
+    // kinematics run each call in one chunk.
+    h->no_recs = h->opt.integratorType == -1;
+    if (h->no_recs) {
+        C = 16;
+        pool = 0;
+    }
     h->rec_cap = (uint32_t)C;
     const uint32_t pages = (uint32_t)std::min<uint64_t>(pool / OVF_PAGE, 0xfffffff0u);
     if (pages > 0) {
@@ -646,6 +655,7 @@ static int run_pipeline(st_handle* h, RunArgs R) {
     A.next_count = h->d_small + 1;
     A.integ = integ;
     A.phys = h->d_phys;
+    A.no_recs = h->no_recs ? 1u : 0u;
     uint32_t n_list = h->n;
     int cur = 0;
     const SpinFn spin =

@@ -70,6 +70,10 @@ struct PipeArgs {
     uint32_t* ovf_link;      // per page: the same particle's next page
     uint32_t* ovf_head;      // pages taken this chunk (zeroed per chunk)
     uint32_t ovf_pages;
+    // trajectory-only runs (integratorType -1): no spin kernel reads the
+    // records, so k_kin writes none and runs each particle to the end of the
+    // call in one chunk
+    uint32_t no_recs;
 };
 
 constexpr uint32_t OVF_PAGE = 256;
@@ -318,7 +322,7 @@ __global__ __launch_bounds__(256) void k_kin(const SimParams P, const DevState D
         const DevState B{A.backup, D.stride};
 #pragma unroll
         for (int f = 0; f < N_KIN_FIELDS; ++f) B.st<uint64_t>(KIN_FIELDS[f], o, D.ld<uint64_t>(KIN_FIELDS[f], o));
-        uint32_t j = 0, lim = A.C, page = OVF_NONE, rpage = OVF_NONE;
+        uint32_t j = 0, lim = A.no_recs ? 0xffffffffu : A.C, page = OVF_NONE, rpage = OVF_NONE;
         const double t0 = L.K.t;
         while (!L.done) {
             if (j == lim) {
@@ -346,7 +350,7 @@ __global__ __launch_bounds__(256) void k_kin(const SimParams P, const DevState D
             double rt;
             d3 rp, rv;
             const uint64_t fl = kin_step(P, F, A.R, A.integ, L, rt, rp, rv);
-            put_rec(rec_at_seq(A, L.pid, j, rpage), rt, rp, rv, fl);
+            if (!A.no_recs) put_rec(rec_at_seq(A, L.pid, j, rpage), rt, rp, rv, fl);
             ++j;
             cnt[0] += (fl & R_EMPTY) ? 0 : 1;
         }

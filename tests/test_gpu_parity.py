@@ -339,3 +339,39 @@ def test_snapshot_blocks_match_snapshots():
             b = np.frombuffer(host[k * bb:(k + 1) * bb].tobytes(), dtype=dt)[0]
             for f in ("t", "pos", "v", "S", "failure", "n_coll", "n_bounce", "n_steps"):
                 assert b[f].tobytes() == s[f].tobytes(), (k, f)
+
+
+@pytest.mark.parametrize("persistent", [False, True])
+@pytest.mark.parametrize("kind", ["neutrons_specular", "he3_gas_diffuse"])
+def test_trajectory_only_matches_oracle(oracle, kind, persistent):
+    """integratorType -1 (trajectory only, src/particle.cpp's loop with no
+    spin integration): the kinematics kernel keeps no segment records and
+    runs each call in one chunk.  Integer state and RNG words bit-exact; the
+    spin stays the initial one; positions within 1e-9 without gas and diffuse
+    walls, and within the reference's own 1-ulp spread with them."""
+    if kind == "neutrons_specular":
+        opt = configs.neutron_cell(n=2048, tf=1.0, ioutInt=0.25, seed=3)
+    else:
+        opt = configs.he3_thermal(n=2048, tf=0.5, ioutInt=0.125, seed=5)
+    opt.integratorType = -1
+    targets = st.output_times(opt)
+    g, gs, gc = run_gpu(opt, targets, persistent=persistent)
+    E = run_oracle(oracle, opt, targets)
+    r = E.snapshot()
+    for k in ("n_bounce", "n_coll", "n_steps", "failure"):
+        bad = np.nonzero(g[k] != r[k])[0]
+        assert bad.size == 0, f"{k} differs for {bad.size} particles, first {bad[:5]}"
+    for k in ("x", "y", "z", "w"):
+        assert np.array_equal(gs[f"rng_{k}"], E.arr["state"][k]), f"rng word {k} differs"
+    assert np.all(gc["segments"] == 0) and np.all(gc["accepted"] == 0)
+    gS, oS = st.xyz(g["S"]), oracle.xyz(r["S"])
+    assert np.array_equal(gS, oS, equal_nan=True), "spin differs"
+    live = np.isfinite(gS).all(1)
+    assert live.mean() > 0.99 and np.all(gS[live] == np.array([opt.yi.x, opt.yi.y, opt.yi.z])), "spin moved"
+    assert np.nanmax(np.abs(g["t"] - r["t"])) < 1e-12
+    dp = np.abs(st.xyz(g["pos"]) - oracle.xyz(r["pos"])).max(1)
+    if kind == "neutrons_specular":
+        assert dp.max() < TOL, dp.max()
+    else:
+        _, cp = ulp_control(oracle, opt, targets)
+        assert np.mean(dp > TOL) <= 2 * np.mean(cp > TOL) + 0.005, (np.mean(dp > TOL), np.mean(cp > TOL))
