@@ -309,6 +309,10 @@ __global__ __launch_bounds__(256) void k_begin_run(const DevState D, uint32_t n,
     list[i] = i;
 }
 
+// waiting lanes that start a wave's diffuse re-emissions (He-3, 1M, 4 x 0.5 s:
+// k_kin 105 ms at 4, 106 at 8, 114 at 16, 108 without the wait); a particle
+// whose walls are mostly diffuse takes its re-emission at once
+constexpr int DIFFUSE_BATCH = 4;
 template <bool BATCH>
 __global__ __launch_bounds__(256) void k_kin(const SimParams P, const DevState D, const PipeArgs A) {
     const uint32_t li = blockIdx.x * blockDim.x + threadIdx.x;
@@ -324,8 +328,13 @@ __global__ __launch_bounds__(256) void k_kin(const SimParams P, const DevState D
         for (int f = 0; f < N_KIN_FIELDS; ++f) B.st<uint64_t>(KIN_FIELDS[f], o, D.ld<uint64_t>(KIN_FIELDS[f], o));
         uint32_t j = 0, lim = A.no_recs ? 0xffffffffu : A.C, page = OVF_NONE, rpage = OVF_NONE;
         const double t0 = L.K.t;
+        KinEvent E;
+        bool pend = false;   // the event's diffuse re-emission waits for more lanes
+        const int batch = F.diffuse < 0.5 ? DIFFUSE_BATCH : 1;
+        double rt;
+        d3 rp, rv;
         while (!L.done) {
-            if (j == lim) {
+            if (!pend && j == lim) {
                 // Chunk depth used up: continue on an overflow page if one is
                 // left and the particle is near its end -- its remaining time
                 // to the last output under twice what this chunk has covered
@@ -347,9 +356,29 @@ __global__ __launch_bounds__(256) void k_kin(const SimParams P, const DevState D
                 page = p;
                 lim += OVF_PAGE;
             }
-            double rt;
-            d3 rp, rv;
-            const uint64_t fl = kin_step(P, F, A.R, A.integ, L, rt, rp, rv);
+            if (!pend) {
+                kin_begin(P, F, A.R, L, E, rt, rp, rv);
+                if (!E.empty) {
+                    if (E.job == JOB_GAS)
+                        gas_job(L.K.v, L.K.next_gas, E.gas_draw, E.Vel, F.tc, F.sqrtKT_m, F.dist, L.rng);
+                    pend = E.job == JOB_DIFFUSE;
+                }
+            }
+            // The diffuse re-emission (acos and two sincos; about a sixth of
+            // this kernel's time, for ~3% of He-3 events) would run on almost
+            // every trip for a lane or two.  It runs once `batch` lanes of the
+            // wave wait for it, or when no running lane has other work;
+            // a waiting lane idles, its random stream untouched meanwhile, so
+            // every lane's events and draws are those of kin_step.
+            {
+                const uint64_t waiting = __ballot(pend), running = __ballot(1);
+                if (pend && (__popcll(waiting) >= batch || waiting == running)) {
+                    diffuse_job(L.K.v, L.K.stop, L.K.wall_hit, E.Vel, L.rng);
+                    pend = false;
+                }
+            }
+            if (pend) continue;
+            const uint64_t fl = kin_end(P, A.R, A.integ, L, E, rt, rp, rv);
             if (!A.no_recs) put_rec(rec_at_seq(A, L.pid, j, rpage), rt, rp, rv, fl);
             ++j;
             cnt[0] += (fl & R_EMPTY) ? 0 : 1;
